@@ -1,0 +1,490 @@
+"""
+curfp_amd — RFP (Rectangular Full Packed) routines for AMD Instinct MI355X.
+
+The Python surface mirrors the reference package ``curfp`` (curfp/__init__.py)
+for the RFP hot path, so ``import curfp_amd as curfp`` is a drop-in there:
+
+    import torch, curfp_amd as curfp
+    n, k, nrhs = 4096, 128, 10
+    A = torch.randn(n, k, device="cuda") / k**0.5
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    B = torch.randn(n, nrhs, device="cuda")
+    curfp.ssfrk(A, C)                 # C = A @ A.T in RFP format
+    curfp.add_to_diagonal(C, 1.0)     # C += I
+    curfp.spftrf(C)                   # Cholesky, in place
+    curfp.spftrs(C, B)                # B := C^{-1} B
+
+Every call goes through the C ABI in include/curfp.h (libcurfp_amd.so) to
+hand-written gfx950 HIP kernels; there is no CPU or PyTorch fallback: when the
+native library is missing the import fails.
+
+Beyond the reference: ``set_math_mode`` (BF16 / split-BF16 MFMA for ssfrk and
+the spftrf updates), ``ssfrk`` on a bfloat16 ``A``, the FP64 generator
+residual ``gen_residual`` and the iterative-refinement solver in
+``curfp_amd.refine``, and the autograd chain in ``curfp_amd.autograd``.
+"""
+from __future__ import annotations
+
+import math
+
+import torch
+
+try:
+    from ._curfp_C import (  # noqa: F401
+        Handle as _HandleBase,
+        OP_N,
+        OP_T,
+        FILL_LOWER,
+        FILL_UPPER,
+        MATH_FP32,
+        MATH_BF16,
+        MATH_BF16X3,
+        KERNEL_CLASSES,
+        ssfrk as _ssfrk_C,
+        ssfrk_bf16 as _ssfrk_bf16_C,
+        spftrf as _spftrf_C,
+        spftrs as _spftrs_C,
+        spftrs_rm as _spftrs_rm_C,
+        strttf as _strttf_C,
+        stfttr as _stfttr_C,
+        sadd_to_diagonal as _sadd_diag_C,
+        sgen_residual as _sgen_residual_C,
+        dsfrk as _dsfrk_C,
+        dpftrf as _dpftrf_C,
+        dpftrs as _dpftrs_C,
+        dpftrs_rm as _dpftrs_rm_C,
+        dstrttf as _dstrttf_C,
+        dstfttr as _dstfttr_C,
+        dadd_to_diagonal as _dadd_diag_C,
+    )
+except ImportError as exc:  # pragma: no cover - exercised only without a build
+    raise ImportError(
+        "curfp_amd native extension is not built; run `python curfp_amd/_build.py` "
+        "(hipcc --offload-arch=gfx950) first"
+    ) from exc
+
+# Enum values of the reference module (curfp/_ext.cpp:869-877) that belong to
+# routines outside the RFP hot path built here.
+NORM_MAX, NORM_ONE, NORM_FRO = 0, 1, 2
+SIDE_LEFT, SIDE_RIGHT = 0, 1
+
+# ---------------------------------------------------------------------------
+# Global handle cache (one handle per device) — curfp/__init__.py:88-106
+# ---------------------------------------------------------------------------
+_handles: dict = {}
+
+
+def _get_handle() -> "Handle":
+    dev = torch.cuda.current_device()
+    h = _handles.get(dev)
+    if h is None:
+        h = Handle()
+        _handles[dev] = h
+    return h
+
+
+def set_stream(stream) -> None:
+    """Bind curfp operations on the current device to ``stream`` (None = default)."""
+    h = _get_handle()
+    h.set_stream_ptr(0 if stream is None else stream.cuda_stream)
+
+
+_MATH_MAP = {"fp32": MATH_FP32, "bf16": MATH_BF16, "bf16x3": MATH_BF16X3}
+
+
+def set_math_mode(mode: str) -> None:
+    """Arithmetic of ssfrk and the spftrf updates on the current device's handle:
+    'fp32' (default), 'bf16' or 'bf16x3' (split-BF16, FP32-accurate products)."""
+    try:
+        m = _MATH_MAP[mode.lower()]
+    except KeyError:
+        raise ValueError(f"math mode must be one of {sorted(_MATH_MAP)}, got {mode!r}")
+    _get_handle().set_math_mode(m)
+
+
+def get_math_mode() -> str:
+    inv = {v: k for k, v in _MATH_MAP.items()}
+    return inv[_get_handle().get_math_mode()]
+
+
+def set_profiling(enable: bool) -> None:
+    """Bracket every library kernel with HIP events (and reset the totals)."""
+    _get_handle().set_profiling(bool(enable))
+
+
+def kernel_stats(reset: bool = False) -> dict:
+    """Per kernel class: launches, algorithmic flops / bytes, summed ms."""
+    h = _get_handle()
+    return {name: h.kernel_stats(cls, reset) for name, cls in KERNEL_CLASSES.items()}
+
+
+class math_mode:
+    """Context manager: ``with curfp.math_mode('bf16'): ...``."""
+
+    def __init__(self, mode: str):
+        self.mode = mode
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = get_math_mode()
+        set_math_mode(self.mode)
+        return self
+
+    def __exit__(self, *exc):
+        set_math_mode(self.prev)
+        return False
+
+
+# ---------------------------------------------------------------------------
+# String -> enum helpers — curfp/__init__.py:112-154
+# ---------------------------------------------------------------------------
+_OP_MAP = {"N": OP_N, "T": OP_T}
+_FILL_MAP = {"L": FILL_LOWER, "U": FILL_UPPER}
+
+
+def _op(s: str) -> int:
+    try:
+        return _OP_MAP[s.upper()]
+    except (KeyError, AttributeError):
+        raise ValueError(f"transr/trans must be 'N' or 'T', got {s!r}")
+
+
+def _fill(s: str) -> int:
+    try:
+        return _FILL_MAP[s.upper()]
+    except (KeyError, AttributeError):
+        raise ValueError(f"uplo must be 'L' or 'U', got {s!r}")
+
+
+def _n_from_rfp(numel: int) -> int:
+    """Recover n from numel == n*(n+1)//2 (curfp/__init__.py:160-167), exactly."""
+    n = (math.isqrt(8 * numel + 1) - 1) // 2
+    if n * (n + 1) // 2 != numel:
+        raise ValueError(f"Tensor has {numel} elements which is not a valid RFP size n*(n+1)//2")
+    return n
+
+
+def _validate(t: torch.Tensor, name: str, dtype=torch.float32) -> None:
+    """curfp/__init__.py:173-188."""
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a CUDA tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    if t.dtype != dtype:
+        name64 = "float64" if dtype == torch.float64 else "float32"
+        raise ValueError(f"{name} must be {name64} (got {t.dtype})")
+
+
+def _validate64(t: torch.Tensor, name: str) -> None:
+    _validate(t, name, torch.float64)
+
+
+# ---------------------------------------------------------------------------
+# RFP geometry (same 8-variant layout as LAPACK; curfp/__init__.py:194-269)
+# ---------------------------------------------------------------------------
+def rfp_blocks(n: int, transr: str = "T", uplo: str = "U"):
+    """(n1, n2, ld, v11, v21, v22): the three blocks of the lower factor as
+    strided views (offset, row stride, col stride) into the RFP array."""
+    lower = uplo.upper() == "L"
+    tn = transr.upper() == "N"
+    if n % 2:
+        n1, n2 = (n - n // 2, n // 2) if lower else (n // 2, n - n // 2)
+        if tn:
+            ld = n
+            o11, o21, o22 = (0, n1, n) if lower else (n2, 0, n1)
+        elif lower:
+            ld, o11, o21, o22 = n1, 0, n1 * n1, 1
+        else:
+            ld, o11, o21, o22 = n2, n2 * n2, 0, n1 * n2
+    else:
+        k = n // 2
+        n1 = n2 = k
+        if tn:
+            ld = n + 1
+            o11, o21, o22 = (1, k + 1, 0) if lower else (k + 1, 0, k)
+        else:
+            ld = k
+            o11, o21, o22 = (k, k * (k + 1), 0) if lower else (k * (k + 1), 0, k * k)
+
+    def view(o, row):
+        return (o, ld, 1) if row else (o, 1, ld)
+
+    return n1, n2, ld, view(o11, not tn), view(o21, tn != lower), view(o22, tn)
+
+
+def rfp_diag_indices(n: int, transr: str = "T", uplo: str = "U", device=None) -> torch.Tensor:
+    """Flat RFP indices of the n diagonal elements (int64), all 8 variants."""
+    if device is None:
+        device = torch.cuda.current_device()
+    n1, n2, ld, v11, _, v22 = rfp_blocks(n, transr, uplo)
+    d1 = v11[0] + torch.arange(n1, device=device, dtype=torch.int64) * (ld + 1)
+    d2 = v22[0] + torch.arange(n2, device=device, dtype=torch.int64) * (ld + 1)
+    return torch.cat([d1, d2])
+
+
+def add_to_diagonal(C: torch.Tensor, value: float, transr: str = "T", uplo: str = "U", n: int = None) -> None:
+    """C[diag] += value (curfp/__init__.py:272-296), as one HIP kernel."""
+    if not C.is_cuda:
+        raise ValueError("C must be a CUDA tensor")
+    if not C.is_contiguous():
+        raise ValueError("C must be contiguous")
+    if C.dtype not in (torch.float32, torch.float64):
+        raise ValueError(f"C must be float32 or float64 (got {C.dtype})")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    fn = _sadd_diag_C if C.dtype == torch.float32 else _dadd_diag_C
+    fn(_get_handle(), _op(transr), _fill(uplo), n, float(value), C.data_ptr())
+
+
+class Handle(_HandleBase):
+    """curfp library handle (stream, math mode, cached workspace)."""
+
+    def set_stream(self, stream) -> None:
+        self.set_stream_ptr(stream.cuda_stream)
+
+
+# ---------------------------------------------------------------------------
+# Raw API — explicit handle, integer enums (curfp/__init__.py:323-405)
+# ---------------------------------------------------------------------------
+def ssfrk_raw(handle, transr, uplo, trans, n, k, alpha, A, lda, beta, C) -> None:
+    _validate(C, "C")
+    if A.dtype == torch.bfloat16:
+        if not A.is_cuda or not A.is_contiguous():
+            raise ValueError("A must be a contiguous CUDA tensor")
+        _ssfrk_bf16_C(handle, transr, uplo, trans, n, k, float(alpha), A.data_ptr(), lda, float(beta), C.data_ptr())
+        return
+    _validate(A, "A")
+    _ssfrk_C(handle, transr, uplo, trans, n, k, float(alpha), A.data_ptr(), lda, float(beta), C.data_ptr())
+
+
+def spftrf_raw(handle, transr, uplo, n, A) -> int:
+    _validate(A, "A")
+    return _spftrf_C(handle, transr, uplo, n, A.data_ptr())
+
+
+def spftrs_raw(handle, transr, uplo, n, nrhs, A, B, ldb) -> None:
+    """B is (n x nrhs) column-major with leading dimension ldb (C ABI layout)."""
+    _validate(A, "A")
+    _validate(B, "B")
+    _spftrs_C(handle, transr, uplo, n, nrhs, A.data_ptr(), B.data_ptr(), ldb)
+
+
+def strttf_raw(handle, transr, uplo, n, A, lda, arf) -> None:
+    _validate(A, "A")
+    _validate(arf, "arf")
+    _strttf_C(handle, transr, uplo, n, A.data_ptr(), lda, arf.data_ptr())
+
+
+def stfttr_raw(handle, transr, uplo, n, arf, A, lda) -> None:
+    _validate(arf, "arf")
+    _validate(A, "A")
+    _stfttr_C(handle, transr, uplo, n, arf.data_ptr(), A.data_ptr(), lda)
+
+
+def dsfrk_raw(handle, transr, uplo, trans, n, k, alpha, A, lda, beta, C) -> None:
+    _validate64(A, "A")
+    _validate64(C, "C")
+    _dsfrk_C(handle, transr, uplo, trans, n, k, float(alpha), A.data_ptr(), lda, float(beta), C.data_ptr())
+
+
+def dpftrf_raw(handle, transr, uplo, n, A) -> int:
+    _validate64(A, "A")
+    return _dpftrf_C(handle, transr, uplo, n, A.data_ptr())
+
+
+def dpftrs_raw(handle, transr, uplo, n, nrhs, A, B, ldb) -> None:
+    _validate64(A, "A")
+    _validate64(B, "B")
+    _dpftrs_C(handle, transr, uplo, n, nrhs, A.data_ptr(), B.data_ptr(), ldb)
+
+
+def dstrttf_raw(handle, transr, uplo, n, A, lda, arf) -> None:
+    _validate64(A, "A")
+    _validate64(arf, "arf")
+    _dstrttf_C(handle, transr, uplo, n, A.data_ptr(), lda, arf.data_ptr())
+
+
+def dstfttr_raw(handle, transr, uplo, n, arf, A, lda) -> None:
+    _validate64(arf, "arf")
+    _validate64(A, "A")
+    _dstfttr_C(handle, transr, uplo, n, arf.data_ptr(), A.data_ptr(), lda)
+
+
+# ---------------------------------------------------------------------------
+# High-level API — global handle, string parameters, shape inference
+# ---------------------------------------------------------------------------
+def _sfrk(A, C, alpha, beta, transr, uplo, trans, n, k, lda, f64):
+    if f64:
+        _validate64(A, "A")
+        _validate64(C, "C")
+    elif A.dtype == torch.bfloat16:
+        _validate(C, "C")
+        if not A.is_cuda or not A.is_contiguous():
+            raise ValueError("A must be a contiguous CUDA tensor")
+    else:
+        _validate(A, "A")
+        _validate(C, "C")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    t = trans.upper()
+    if k is None:
+        k = A.shape[1] if t == "T" else A.shape[0]
+    if lda is None:
+        lda = A.shape[1] if t == "T" else A.shape[0]
+    if f64:
+        fn = _dsfrk_C
+    else:
+        fn = _ssfrk_bf16_C if A.dtype == torch.bfloat16 else _ssfrk_C
+    fn(_get_handle(), _op(transr), _fill(uplo), _op(trans), n, k, float(alpha), A.data_ptr(), lda,
+       float(beta), C.data_ptr())
+
+
+def ssfrk(A, C, *, alpha=1.0, beta=0.0, transr="T", uplo="U", trans="T", n=None, k=None, lda=None) -> None:
+    """C := alpha*A@A.T + beta*C into RFP storage (curfp/__init__.py:410-465).
+
+    trans='T': A is row-major (n, k); trans='N': A is column-major n x k
+    (i.e. a row-major (k, n) tensor).  A may be bfloat16 (BF16 MFMA with FP32
+    accumulation; C stays float32)."""
+    _sfrk(A, C, alpha, beta, transr, uplo, trans, n, k, lda, False)
+
+
+def dsfrk(A, C, *, alpha=1.0, beta=0.0, transr="T", uplo="U", trans="T", n=None, k=None, lda=None) -> None:
+    """FP64 ssfrk (curfp/__init__.py:1439-1474)."""
+    _sfrk(A, C, alpha, beta, transr, uplo, trans, n, k, lda, True)
+
+
+def _pftrf(C, n, transr, uplo, check, f64):
+    (_validate64 if f64 else _validate)(C, "C")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    fn = _dpftrf_C if f64 else _spftrf_C
+    info = fn(_get_handle(), _op(transr), _fill(uplo), n, C.data_ptr())
+    if check and info != 0:
+        raise torch.linalg.LinAlgError(
+            f"{'dpftrf' if f64 else 'spftrf'}: matrix is not positive definite "
+            f"(leading minor of order {info} is not positive definite)"
+        )
+    return info
+
+
+def spftrf(C, *, n=None, transr="T", uplo="U", check=True) -> int:
+    """In-place RFP Cholesky (curfp/__init__.py:468-501); returns info."""
+    return _pftrf(C, n, transr, uplo, check, False)
+
+
+def dpftrf(C, *, n=None, transr="T", uplo="U", check=True) -> int:
+    """FP64 spftrf (curfp/__init__.py:1511-1529)."""
+    return _pftrf(C, n, transr, uplo, check, True)
+
+
+def _pftrs(C, B, n, nrhs, transr, uplo, f64):
+    v = _validate64 if f64 else _validate
+    v(C, "C")
+    v(B, "B")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    squeezed = B.ndim == 1
+    Bv = B.unsqueeze(1) if squeezed else B
+    if nrhs is None:
+        nrhs = Bv.shape[1]
+    if Bv.shape[0] != n:
+        raise ValueError(f"B has {Bv.shape[0]} rows but n={n}")
+    # B is (n, nrhs) row-major: solved in place through the row-major entry
+    # point, no transposed copies (the reference copies twice, :548 and :562).
+    fn = _dpftrs_rm_C if f64 else _spftrs_rm_C
+    fn(_get_handle(), _op(transr), _fill(uplo), n, nrhs, C.data_ptr(), Bv.data_ptr(), max(1, Bv.shape[1]))
+
+
+def spftrs(C, B, *, n=None, nrhs=None, transr="T", uplo="U") -> None:
+    """Solve C X = B in place on B, shape (n, nrhs) or (n,) (curfp/__init__.py:504-565)."""
+    _pftrs(C, B, n, nrhs, transr, uplo, False)
+
+
+def dpftrs(C, B, *, n=None, nrhs=None, transr="T", uplo="U") -> None:
+    """FP64 spftrs (curfp/__init__.py:1547-1581)."""
+    _pftrs(C, B, n, nrhs, transr, uplo, True)
+
+
+def _trttf(A, transr, uplo, n, f64):
+    (_validate64 if f64 else _validate)(A, "A")
+    if A.ndim != 2 or A.shape[0] != A.shape[1]:
+        raise ValueError(f"A must be a square 2-D tensor, got shape {tuple(A.shape)}")
+    if n is None:
+        n = A.shape[0]
+    arf = torch.empty(n * (n + 1) // 2, dtype=A.dtype, device=A.device)
+    fn = _dstrttf_C if f64 else _strttf_C
+    fn(_get_handle(), _op(transr), _fill(uplo), n, A.data_ptr(), A.shape[1], arf.data_ptr())
+    return arf
+
+
+def _tfttr(arf, transr, uplo, n, f64):
+    (_validate64 if f64 else _validate)(arf, "arf")
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    A = torch.zeros(n, n, dtype=arf.dtype, device=arf.device)
+    fn = _dstfttr_C if f64 else _stfttr_C
+    fn(_get_handle(), _op(transr), _fill(uplo), n, arf.data_ptr(), A.data_ptr(), n)
+    return A
+
+
+def strttf(A, *, transr="T", uplo="U", n=None) -> torch.Tensor:
+    """Dense (n, n) triangle -> new RFP tensor (curfp/__init__.py:719-750)."""
+    return _trttf(A, transr, uplo, n, False)
+
+
+def stfttr(arf, *, transr="T", uplo="U", n=None) -> torch.Tensor:
+    """RFP -> new dense (n, n) tensor, other triangle zero (curfp/__init__.py:753-781)."""
+    return _tfttr(arf, transr, uplo, n, False)
+
+
+def dstrttf(A, *, transr="T", uplo="U", n=None) -> torch.Tensor:
+    """FP64 strttf (curfp/__init__.py:1347-1375)."""
+    return _trttf(A, transr, uplo, n, True)
+
+
+def dstfttr(arf, *, transr="T", uplo="U", n=None) -> torch.Tensor:
+    """FP64 stfttr (curfp/__init__.py:1378-1403)."""
+    return _tfttr(arf, transr, uplo, n, True)
+
+
+def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R=None) -> torch.Tensor:
+    """FP64 residual R = B - (alpha*A@A.T + lam*I) @ X from the FP32 generator A.
+
+    A: float32 (n, k) row-major (trans='T').  X, B: float64 (n, nrhs) — passed
+    to the kernels as column-major nrhs x n... i.e. each is handled through its
+    transpose, which is column-major n x nrhs with leading dimension n when
+    X.t() is contiguous.  For simplicity X and B here are column-major (n,
+    nrhs) tensors: pass ``X.t().contiguous().t()``-style storage, or use
+    :mod:`curfp_amd.refine`, which manages the layouts.
+    """
+    _validate(A, "A")
+    n = A.shape[0] if trans.upper() == "T" else A.shape[1]
+    k = A.shape[1] if trans.upper() == "T" else A.shape[0]
+    lda = A.shape[1] if trans.upper() == "T" else A.shape[0]
+    nrhs = X.shape[1]
+    for t, nm in ((X, "X"), (B, "B")):
+        if t.dtype != torch.float64 or not t.is_cuda or t.shape != (n, nrhs) or t.stride() != (1, n):
+            raise ValueError(f"{nm} must be a float64 CUDA (n, nrhs) column-major tensor")
+    if R is None:
+        R = torch.empty_strided((n, nrhs), (1, n), dtype=torch.float64, device=A.device)
+    _sgen_residual_C(_get_handle(), _op(trans), n, k, nrhs, float(alpha), A.data_ptr(), lda, float(lam),
+                     X.data_ptr(), n, B.data_ptr(), n, R.data_ptr(), n)
+    return R
+
+
+# ---------------------------------------------------------------------------
+# Reference names outside the RFP hot path (curfp/__init__.py:42-80)
+# ---------------------------------------------------------------------------
+def _not_built(name):
+    def fn(*args, **kwargs):
+        raise NotImplementedError(f"curfp_amd: {name} is outside the RFP hot path built for MI355X")
+
+    fn.__name__ = name
+    return fn
+
+
+for _name in ("spftri", "ssfmv", "slansf", "spfcon", "ssfr", "ssfr2", "ssfr2k", "ssfmm", "spftrf_rcond",
+              "dpftri", "dsfmv", "dlansf", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dsfmm", "dpftrf_rcond"):
+    globals()[_name] = _not_built(_name)
+    globals()[_name + "_raw"] = _not_built(_name + "_raw")
+del _name

@@ -1,0 +1,165 @@
+/*
+ * pyext.cpp — the `_curfp_C` Python extension: a thin pybind11 layer over
+ * the C ABI in include/curfp.h, with the same surface as the reference's
+ * curfp/_ext.cpp:30-878 for the routines built here (Handle with
+ * set_stream_ptr, one function per C entry, tensors passed as integer device
+ * pointers, enums as ints, a non-zero status raised as
+ * RuntimeError("curfp error: status=N")).
+ */
+#include <pybind11/pybind11.h>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include "../../include/curfp.h"
+
+namespace py = pybind11;
+
+static void check_status(curfpStatus_t s)
+{
+    if (s != CURFP_STATUS_SUCCESS)
+        throw std::runtime_error("curfp error: status=" + std::to_string((int)s));
+}
+
+class Handle {
+    curfpHandle_t h_ = nullptr;
+
+public:
+    Handle() { check_status(curfpCreate(&h_)); }
+    ~Handle() { if (h_) curfpDestroy(h_); }
+    Handle(const Handle &) = delete;
+    Handle &operator=(const Handle &) = delete;
+
+    void set_stream_ptr(int64_t ptr) { check_status(curfpSetStream(h_, reinterpret_cast<hipStream_t>(ptr))); }
+    int64_t get_stream_ptr() const
+    {
+        hipStream_t s = nullptr;
+        check_status(curfpGetStream(h_, &s));
+        return reinterpret_cast<int64_t>(s);
+    }
+    void set_math_mode(int mode) { check_status(curfpSetMathMode(h_, static_cast<curfpMathMode_t>(mode))); }
+    int get_math_mode() const
+    {
+        curfpMathMode_t m = CURFP_MATH_FP32;
+        check_status(curfpGetMathMode(h_, &m));
+        return (int)m;
+    }
+    void set_profiling(bool on) { check_status(curfpSetProfiling(h_, on ? 1 : 0)); }
+    py::dict kernel_stats(int cls, bool reset)
+    {
+        int64_t launches = 0;
+        double flops = 0, bytes = 0, ms = 0;
+        check_status(curfpGetKernelStats(h_, static_cast<curfpKernelClass_t>(cls), &launches, &flops, &bytes, &ms,
+                                         reset ? 1 : 0));
+        py::dict d;
+        d["launches"] = launches;
+        d["flops"] = flops;
+        d["bytes"] = bytes;
+        d["ms"] = ms;
+        return d;
+    }
+    curfpHandle_t get() const { return h_; }
+};
+
+template <typename T> static T *P(int64_t p) { return reinterpret_cast<T *>(p); }
+#define OPS(x) static_cast<curfpOperation_t>(x)
+#define FILL(x) static_cast<curfpFillMode_t>(x)
+
+PYBIND11_MODULE(_curfp_C, m)
+{
+    m.doc() = "curfp C ABI for MI355X (gfx950), HIP kernels";
+
+    py::class_<Handle>(m, "Handle")
+        .def(py::init<>())
+        .def("set_stream_ptr", &Handle::set_stream_ptr)
+        .def("get_stream_ptr", &Handle::get_stream_ptr)
+        .def("set_math_mode", &Handle::set_math_mode)
+        .def("get_math_mode", &Handle::get_math_mode)
+        .def("set_profiling", &Handle::set_profiling)
+        .def("kernel_stats", &Handle::kernel_stats, py::arg("cls"), py::arg("reset") = false)
+        .def("__enter__", [](Handle &h) -> Handle & { return h; }, py::return_value_policy::reference)
+        .def("__exit__", [](Handle &, py::object, py::object, py::object) {});
+
+    m.attr("OP_N") = (int)CURFP_OP_N;
+    m.attr("OP_T") = (int)CURFP_OP_T;
+    m.attr("FILL_LOWER") = (int)CURFP_FILL_MODE_LOWER;
+    m.attr("FILL_UPPER") = (int)CURFP_FILL_MODE_UPPER;
+    m.attr("MATH_FP32") = (int)CURFP_MATH_FP32;
+    m.attr("MATH_BF16") = (int)CURFP_MATH_BF16;
+    m.attr("MATH_BF16X3") = (int)CURFP_MATH_BF16X3;
+    py::dict kc;
+    kc["gemm_f32"] = (int)CURFP_KERNEL_GEMM_F32;
+    kc["gemm_bf16"] = (int)CURFP_KERNEL_GEMM_BF16;
+    kc["gemm_f64"] = (int)CURFP_KERNEL_GEMM_F64;
+    kc["potrf_tile"] = (int)CURFP_KERNEL_POTRF_TILE;
+    kc["trtri"] = (int)CURFP_KERNEL_TRTRI;
+    kc["pack"] = (int)CURFP_KERNEL_PACK;
+    kc["diag"] = (int)CURFP_KERNEL_DIAG;
+    kc["other"] = (int)CURFP_KERNEL_OTHER;
+    m.attr("KERNEL_CLASSES") = kc;
+
+    /* ---- FP32 ---- */
+    m.def("ssfrk", [](Handle &h, int transr, int uplo, int trans, int n, int k, float alpha, int64_t A, int lda,
+                      float beta, int64_t C) {
+        check_status(curfpSsfrk(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, &alpha, P<const float>(A), lda,
+                                &beta, P<float>(C)));
+    });
+    m.def("ssfrk_bf16", [](Handle &h, int transr, int uplo, int trans, int n, int k, float alpha, int64_t A,
+                           int lda, float beta, int64_t C) {
+        check_status(curfpSsfrkBf16(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, &alpha,
+                                    P<const uint16_t>(A), lda, &beta, P<float>(C)));
+    });
+    m.def("spftrf", [](Handle &h, int transr, int uplo, int n, int64_t A) {
+        int info = 0;
+        check_status(curfpSpftrf(h.get(), OPS(transr), FILL(uplo), n, P<float>(A), &info));
+        return info;
+    });
+    m.def("spftrs", [](Handle &h, int transr, int uplo, int n, int nrhs, int64_t A, int64_t B, int ldb) {
+        check_status(curfpSpftrs(h.get(), OPS(transr), FILL(uplo), n, nrhs, P<const float>(A), P<float>(B), ldb));
+    });
+    m.def("spftrs_rm", [](Handle &h, int transr, int uplo, int n, int nrhs, int64_t A, int64_t B, int ldb) {
+        check_status(
+            curfpSpftrsRowMajor(h.get(), OPS(transr), FILL(uplo), n, nrhs, P<const float>(A), P<float>(B), ldb));
+    });
+    m.def("strttf", [](Handle &h, int transr, int uplo, int n, int64_t A, int lda, int64_t arf) {
+        check_status(curfpSstrttf(h.get(), OPS(transr), FILL(uplo), n, P<const float>(A), lda, P<float>(arf)));
+    });
+    m.def("stfttr", [](Handle &h, int transr, int uplo, int n, int64_t arf, int64_t A, int lda) {
+        check_status(curfpSstfttr(h.get(), OPS(transr), FILL(uplo), n, P<const float>(arf), P<float>(A), lda));
+    });
+    m.def("sadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, float value, int64_t arf) {
+        check_status(curfpSaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<float>(arf)));
+    });
+    m.def("sgen_residual", [](Handle &h, int trans, int n, int k, int nrhs, double alpha, int64_t A, int lda,
+                              double lambda, int64_t X, int ldx, int64_t B, int ldb, int64_t R, int ldr) {
+        check_status(curfpSgenResidual(h.get(), OPS(trans), n, k, nrhs, &alpha, P<const float>(A), lda, &lambda,
+                                       P<const double>(X), ldx, P<const double>(B), ldb, P<double>(R), ldr));
+    });
+
+    /* ---- FP64 ---- */
+    m.def("dsfrk", [](Handle &h, int transr, int uplo, int trans, int n, int k, double alpha, int64_t A, int lda,
+                      double beta, int64_t C) {
+        check_status(curfpDsfrk(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, &alpha, P<const double>(A), lda,
+                                &beta, P<double>(C)));
+    });
+    m.def("dpftrf", [](Handle &h, int transr, int uplo, int n, int64_t A) {
+        int info = 0;
+        check_status(curfpDpftrf(h.get(), OPS(transr), FILL(uplo), n, P<double>(A), &info));
+        return info;
+    });
+    m.def("dpftrs", [](Handle &h, int transr, int uplo, int n, int nrhs, int64_t A, int64_t B, int ldb) {
+        check_status(curfpDpftrs(h.get(), OPS(transr), FILL(uplo), n, nrhs, P<const double>(A), P<double>(B), ldb));
+    });
+    m.def("dpftrs_rm", [](Handle &h, int transr, int uplo, int n, int nrhs, int64_t A, int64_t B, int ldb) {
+        check_status(
+            curfpDpftrsRowMajor(h.get(), OPS(transr), FILL(uplo), n, nrhs, P<const double>(A), P<double>(B), ldb));
+    });
+    m.def("dstrttf", [](Handle &h, int transr, int uplo, int n, int64_t A, int lda, int64_t arf) {
+        check_status(curfpDstrttf(h.get(), OPS(transr), FILL(uplo), n, P<const double>(A), lda, P<double>(arf)));
+    });
+    m.def("dstfttr", [](Handle &h, int transr, int uplo, int n, int64_t arf, int64_t A, int lda) {
+        check_status(curfpDstfttr(h.get(), OPS(transr), FILL(uplo), n, P<const double>(arf), P<double>(A), lda));
+    });
+    m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
+        check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
+    });
+}

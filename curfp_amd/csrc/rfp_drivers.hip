@@ -1,0 +1,694 @@
+/*
+ * rfp_drivers.hip — host drivers behind the C ABI (include/curfp.h).
+ *
+ * Every routine works on the three strided views of the RFP rectangle
+ * (rfp_geom in rfp_internal.h) and issues hand-written gfx950 kernels on the
+ * handle's stream:
+ *
+ *   ssfrk  : three view GEMMs (lower-triangular SYRK on v11 and v22, full
+ *            GEMM on v21) — the reference's 2x cublasSsyrk + cublasSgemm
+ *            (src/curfp_ssfrk.cpp:193-209);
+ *   spftrf : LAPACK's RFP sequence POTRF(v11) -> TRSM(v21) -> SYRK(v22) ->
+ *            POTRF(v22) (src/curfp_spftrf.cpp:256-279), with the two POTRFs
+ *            and the TRSM themselves blocked recursively down to leaf tiles
+ *            whose Cholesky + inverse run in one workgroup; every update
+ *            above the leaves is an MFMA GEMM;
+ *   spftrs : forward  L11 y1 = b1, b2 -= L21 y1, L22 y2 = b2,
+ *            backward L22^T x2 = y2, b1 -= L21^T x2, L11^T x1 = b1
+ *            (src/curfp_spftrs.cpp:258-299), each triangular solve blocked
+ *            recursively with the leaf solves done as GEMMs with the leaf
+ *            inverses.
+ *
+ * spftrf accumulates `info` on the device and reads it back once, at the end
+ * (the reference synchronises twice, src/curfp_spftrf.cpp:258-260, :280-282).
+ * Workspace (leaf inverses, split-K partials) is cached in the handle instead
+ * of being allocated per call (src/curfp_spftrf.cpp:231-232).
+ */
+#include "rfp_internal.h"
+#include <climits>
+#include <new>
+
+/* ------------------------------------------------------------------------ */
+/* Handle                                                                    */
+/* ------------------------------------------------------------------------ */
+
+static curfpStatus_t st_of(hipError_t e)
+{
+    if (e == hipSuccess) return CURFP_STATUS_SUCCESS;
+    if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return CURFP_STATUS_ALLOC_FAILED;
+    if (e == hipErrorInvalidValue) return CURFP_STATUS_INVALID_VALUE;
+    return CURFP_STATUS_EXECUTION_FAILED;
+}
+
+#define RFP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t _e = (expr);                         \
+        if (_e != hipSuccess) return _e;                \
+    } while (0)
+
+#define RFP_CHECK_HANDLE(h)                             \
+    do {                                                \
+        if (!(h)) return CURFP_STATUS_NOT_INITIALIZED;  \
+    } while (0)
+
+extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
+{
+    if (!handle) return CURFP_STATUS_INVALID_VALUE;
+    curfpContext *ctx = new (std::nothrow) curfpContext();
+    if (!ctx) return CURFP_STATUS_ALLOC_FAILED;
+    ctx->stream = nullptr;
+    ctx->math = CURFP_MATH_FP32;
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+    ctx->d_info = nullptr;
+    ctx->prof = false;
+    for (RfpKernelStat &st : ctx->prof_stats) st = RfpKernelStat{0, 0.0, 0.0, 0.0};
+    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return CURFP_STATUS_NOT_INITIALIZED; }
+    if (hipMalloc((void **)&ctx->d_info, sizeof(int)) != hipSuccess) { delete ctx; return CURFP_STATUS_ALLOC_FAILED; }
+    *handle = ctx;
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpDestroy(curfpHandle_t handle)
+{
+    RFP_CHECK_HANDLE(handle);
+    (void)hipStreamSynchronize(handle->stream);
+    for (const RfpProfRecord &r : handle->prof_pending) {
+        (void)hipEventDestroy(r.e0);
+        (void)hipEventDestroy(r.e1);
+    }
+    for (hipEvent_t e : handle->prof_pool) (void)hipEventDestroy(e);
+    if (handle->ws) (void)hipFree(handle->ws);
+    if (handle->d_info) (void)hipFree(handle->d_info);
+    delete handle;
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Kernel profiling                                                          */
+/* ------------------------------------------------------------------------ */
+
+static hipEvent_t prof_event(curfpHandle_t h)
+{
+    if (!h->prof_pool.empty()) {
+        hipEvent_t e = h->prof_pool.back();
+        h->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+/* Runs `launch` (which enqueues one kernel on h->stream); when profiling is
+ * on, brackets it with events and records its algorithmic work. */
+template <typename F>
+static hipError_t timed(curfpHandle_t h, int cls, double flops, double bytes, F &&launch)
+{
+    if (!h->prof) return launch();
+    hipEvent_t e0 = prof_event(h), e1 = prof_event(h);
+    if (!e0 || !e1) return launch();
+    (void)hipEventRecord(e0, h->stream);
+    hipError_t e = launch();
+    (void)hipEventRecord(e1, h->stream);
+    h->prof_pending.push_back(RfpProfRecord{cls, flops, bytes, e0, e1});
+    return e;
+}
+
+static hipError_t prof_flush(curfpHandle_t h)
+{
+    if (h->prof_pending.empty()) return hipSuccess;
+    RFP_TRY(hipStreamSynchronize(h->stream));
+    for (const RfpProfRecord &r : h->prof_pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+            RfpKernelStat &st = h->prof_stats[r.cls];
+            st.launches += 1;
+            st.flops += r.flops;
+            st.bytes += r.bytes;
+            st.ms += ms;
+        }
+        h->prof_pool.push_back(r.e0);
+        h->prof_pool.push_back(r.e1);
+    }
+    h->prof_pending.clear();
+    return hipSuccess;
+}
+
+extern "C" curfpStatus_t curfpSetProfiling(curfpHandle_t handle, int enable)
+{
+    RFP_CHECK_HANDLE(handle);
+    hipError_t e = prof_flush(handle);
+    if (e != hipSuccess) return st_of(e);
+    handle->prof = enable != 0;
+    for (RfpKernelStat &st : handle->prof_stats) st = RfpKernelStat{0, 0.0, 0.0, 0.0};
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelClass_t cls, int64_t *launches,
+                                             double *flops, double *bytes, double *milliseconds, int reset)
+{
+    RFP_CHECK_HANDLE(handle);
+    if ((int)cls < 0 || (int)cls >= CURFP_KERNEL_CLASSES) return CURFP_STATUS_INVALID_VALUE;
+    hipError_t e = prof_flush(handle);
+    if (e != hipSuccess) return st_of(e);
+    const RfpKernelStat &st = handle->prof_stats[cls];
+    if (launches) *launches = st.launches;
+    if (flops) *flops = st.flops;
+    if (bytes) *bytes = st.bytes;
+    if (milliseconds) *milliseconds = st.ms;
+    if (reset) handle->prof_stats[cls] = RfpKernelStat{0, 0.0, 0.0, 0.0};
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpSetStream(curfpHandle_t handle, hipStream_t stream)
+{
+    RFP_CHECK_HANDLE(handle);
+    handle->stream = stream;
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpGetStream(curfpHandle_t handle, hipStream_t *stream)
+{
+    RFP_CHECK_HANDLE(handle);
+    if (!stream) return CURFP_STATUS_INVALID_VALUE;
+    *stream = handle->stream;
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpSetMathMode(curfpHandle_t handle, curfpMathMode_t mode)
+{
+    RFP_CHECK_HANDLE(handle);
+    if (mode != CURFP_MATH_FP32 && mode != CURFP_MATH_BF16 && mode != CURFP_MATH_BF16X3)
+        return CURFP_STATUS_INVALID_VALUE;
+    handle->math = mode;
+    return CURFP_STATUS_SUCCESS;
+}
+
+extern "C" curfpStatus_t curfpGetMathMode(curfpHandle_t handle, curfpMathMode_t *mode)
+{
+    RFP_CHECK_HANDLE(handle);
+    if (!mode) return CURFP_STATUS_INVALID_VALUE;
+    *mode = handle->math;
+    return CURFP_STATUS_SUCCESS;
+}
+
+hipError_t rfp_workspace(curfpHandle_t h, size_t bytes, void **out)
+{
+    if (bytes == 0) bytes = 256;
+    if (h->ws_bytes < bytes) {
+        /* the old buffer may still be in use by queued work */
+        RFP_TRY(hipStreamSynchronize(h->stream));
+        if (h->ws) { (void)hipFree(h->ws); h->ws = nullptr; h->ws_bytes = 0; }
+        const size_t sz = (bytes + (1u << 20) - 1) & ~((size_t)(1u << 20) - 1);
+        RFP_TRY(hipMalloc(&h->ws, sz));
+        h->ws_bytes = sz;
+    }
+    *out = h->ws;
+    return hipSuccess;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Precision dispatch                                                        */
+/* ------------------------------------------------------------------------ */
+
+template <typename T> struct Prec;
+template <> struct Prec<float> {
+    static hipError_t gemm(const GemmArgs &g, RfpMath m, hipStream_t s) { return launch_gemm_f32(g, m, s); }
+    static hipError_t potrf_tile(float *a, RfpView v, int m, int *info, int base, float *inv, hipStream_t s)
+    { return launch_potrf_tile_f32(a, v, m, info, base, inv, s); }
+    static hipError_t trtri(const float *a, RfpView v, int64_t m, float *inv, hipStream_t s)
+    { return launch_trtri_tiles_f32(a, v, m, inv, s); }
+    static hipError_t trttf(const RfpGeom &g, bool lo, const float *A, int64_t lda, float *arf, hipStream_t s)
+    { return launch_trttf_f32(g, lo, A, lda, arf, s); }
+    static hipError_t tfttr(const RfpGeom &g, bool lo, const float *arf, float *A, int64_t lda, hipStream_t s)
+    { return launch_tfttr_f32(g, lo, arf, A, lda, s); }
+    static hipError_t add_diag(const RfpGeom &g, float v, float *arf, hipStream_t s)
+    { return launch_add_diag_f32(g, v, arf, s); }
+    static constexpr bool f64 = false;
+};
+template <> struct Prec<double> {
+    static hipError_t gemm(const GemmArgs &g, RfpMath, hipStream_t s) { return launch_gemm_f64(g, s); }
+    static hipError_t potrf_tile(double *a, RfpView v, int m, int *info, int base, double *inv, hipStream_t s)
+    { return launch_potrf_tile_f64(a, v, m, info, base, inv, s); }
+    static hipError_t trtri(const double *a, RfpView v, int64_t m, double *inv, hipStream_t s)
+    { return launch_trtri_tiles_f64(a, v, m, inv, s); }
+    static hipError_t trttf(const RfpGeom &g, bool lo, const double *A, int64_t lda, double *arf, hipStream_t s)
+    { return launch_trttf_f64(g, lo, A, lda, arf, s); }
+    static hipError_t tfttr(const RfpGeom &g, bool lo, const double *arf, double *A, int64_t lda, hipStream_t s)
+    { return launch_tfttr_f64(g, lo, arf, A, lda, s); }
+    static hipError_t add_diag(const RfpGeom &g, double v, double *arf, hipStream_t s)
+    { return launch_add_diag_f64(g, v, arf, s); }
+    static constexpr bool f64 = true;
+};
+
+/* Algorithmic flops of one view GEMM: 2*M*N*K, or M*(M+1)*K for a
+ * lower-triangular (SYRK) output. */
+static double gemm_flops(const GemmArgs &g)
+{
+    return g.lower ? (double)g.M * (double)(g.M + 1) * (double)g.K : 2.0 * g.M * (double)g.N * (double)g.K;
+}
+
+template <typename T>
+static hipError_t run_gemm(curfpHandle_t h, const GemmArgs &g, RfpMath m)
+{
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    const int cls = Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64
+                                 : ((m == RFP_MATH_F32 && !g.a_bf16) ? CURFP_KERNEL_GEMM_F32 : CURFP_KERNEL_GEMM_BF16);
+    return timed(h, cls, gemm_flops(g), 0.0, [&] { return Prec<T>::gemm(g, m, h->stream); });
+}
+
+static RfpMath math_of(curfpHandle_t h)
+{
+    switch (h->math) {
+    case CURFP_MATH_BF16: return RFP_MATH_BF16;
+    case CURFP_MATH_BF16X3: return RFP_MATH_BF16X3;
+    default: return RFP_MATH_F32;
+    }
+}
+
+static inline int64_t leaf_split(int64_t m, int64_t leaf)
+{
+    const int64_t nb = (m + leaf - 1) / leaf;
+    return leaf * ((nb + 1) / 2);
+}
+
+/* ------------------------------------------------------------------------ */
+/* ssfrk                                                                     */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo,
+                               curfpOperation_t trans, int n, int k, const T *alpha,
+                               const void *A, int lda, const T *beta, T *C, bool a_bf16)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || k < 0 || lda < 1) return CURFP_STATUS_INVALID_VALUE;
+    if (!alpha || !beta) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) || (trans != CURFP_OP_N && trans != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    const T a = *alpha, b = *beta;
+    if ((a == T(0) || k == 0) && b == T(1)) return CURFP_STATUS_SUCCESS;
+    if (!C) return CURFP_STATUS_INVALID_VALUE;
+    const int64_t nt = (int64_t)n * (n + 1) / 2;
+    if (a == T(0) && b == T(0)) return st_of(launch_fill_zero(C, (size_t)nt * sizeof(T), h->stream));
+    const int64_t K = (a == T(0)) ? 0 : k;     /* alpha == 0: A is never read */
+    if (K > 0 && !A) return CURFP_STATUS_INVALID_VALUE;
+
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    /* row i of op(A) (n x k): trans = N -> A[i + p*lda], trans = T -> A[p + i*lda] */
+    const RfpView av = (trans == CURFP_OP_N) ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
+    const RfpView a1 = av, a2 = av.sub(g.n1, 0);
+    const RfpMath m = a_bf16 ? (h->math == CURFP_MATH_FP32 ? RFP_MATH_BF16 : math_of(h)) : math_of(h);
+
+    struct Blk { RfpView c, x, y; int64_t M, N; bool lower; } blk[3] = {
+        {g.v11, a1, a1, g.n1, g.n1, true},
+        {g.v21, a2, a1, g.n2, g.n1, false},
+        {g.v22, a2, a2, g.n2, g.n2, true},
+    };
+    for (const Blk &bk : blk) {
+        if (bk.M <= 0 || bk.N <= 0) continue;
+        GemmArgs ga = gemm_args();
+        ga.a = A; ga.va = bk.x;
+        ga.b = A; ga.vb = bk.y;
+        ga.c = C; ga.vc = bk.c;
+        ga.M = bk.M; ga.N = bk.N; ga.K = K;
+        ga.alpha = (double)a; ga.beta = (double)b;
+        ga.lower = bk.lower;
+        ga.a_bf16 = a_bf16;
+        hipError_t e = run_gemm<T>(h, ga, m);
+        if (e != hipSuccess) return st_of(e);
+    }
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* spftrf                                                                    */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+struct Factor {
+    curfpHandle_t h;
+    T *a;          /* RFP array */
+    T *inv;        /* leaf inverse slots */
+    int64_t leaf;
+    RfpMath math;  /* arithmetic of the SYRK / GEMM updates */
+
+    hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t N,
+                    int64_t K, double alpha, double beta, bool lower, int inplace, RfpMath m)
+    {
+        GemmArgs g = gemm_args();
+        g.a = x; g.va = vx;
+        g.b = y; g.vb = vy;
+        g.c = a; g.vc = c;
+        g.M = M; g.N = N; g.K = K;
+        g.alpha = alpha; g.beta = beta;
+        g.lower = lower;
+        g.inplace = inplace;
+        return run_gemm<T>(h, g, m);
+    }
+
+    /* B (mb x m) := B * L^{-T}, L = lower-triangular view (m x m) whose
+     * leaf inverses start at `slot` */
+    hipError_t trsm_rt(RfpView B, int64_t mb, RfpView L, int64_t m, int64_t slot)
+    {
+        if (m <= 0 || mb <= 0) return hipSuccess;
+        if (m <= leaf)
+            return gemm(B, a, B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, mb, m, m, 1.0, 0.0,
+                        false, 1, RFP_MATH_F32);
+        const int64_t m1 = leaf_split(m, leaf);
+        RFP_TRY(trsm_rt(B, mb, L, m1, slot));
+        RFP_TRY(gemm(B.sub(0, m1), a, B, a, L.sub(m1, 0), mb, m - m1, m1, -1.0, 1.0, false, 0, math));
+        return trsm_rt(B.sub(0, m1), mb, L.sub(m1, m1), m - m1, slot + m1 / leaf);
+    }
+
+    hipError_t potrf2(RfpView V11, RfpView V21, RfpView V22, int64_t m1, int64_t m2, int64_t goff,
+                      int64_t slot1, int64_t slot2)
+    {
+        RFP_TRY(potrf(V11, m1, goff, slot1));
+        RFP_TRY(trsm_rt(V21, m2, V11, m1, slot1));
+        if (m1 > 0 && m2 > 0)
+            RFP_TRY(gemm(V22, a, V21, a, V21, m2, m2, m1, -1.0, 1.0, true, 0, math));
+        return potrf(V22, m2, goff + m1, slot2);
+    }
+
+    hipError_t potrf(RfpView V, int64_t m, int64_t goff, int64_t slot)
+    {
+        if (m <= 0) return hipSuccess;
+        if (m <= leaf)
+            return timed(h, CURFP_KERNEL_POTRF_TILE, (double)m * m * m / 3.0, 0.0, [&] {
+                return Prec<T>::potrf_tile(a, V, (int)m, h->d_info, (int)goff, inv + slot * leaf * leaf, h->stream);
+            });
+        const int64_t m1 = leaf_split(m, leaf);
+        return potrf2(V, V.sub(m1, 0), V.sub(m1, m1), m1, m - m1, goff, slot, slot + m1 / leaf);
+    }
+};
+
+template <typename T>
+static curfpStatus_t pftrf_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                T *A, int *info)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || !info) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    *info = 0;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!A) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const int64_t leaf = rfp_leaf(Prec<T>::f64);
+    const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, (size_t)(s1 + s2) * leaf * leaf * sizeof(T), &ws);
+    if (e != hipSuccess) return st_of(e);
+    e = launch_set_int(h->d_info, INT_MAX, h->stream);
+    if (e != hipSuccess) return st_of(e);
+    Factor<T> f{h, A, (T *)ws, leaf, Prec<T>::f64 ? RFP_MATH_F64 : math_of(h)};
+    e = f.potrf2(g.v11, g.v21, g.v22, g.n1, g.n2, 0, 0, s1);
+    if (e != hipSuccess) return st_of(e);
+    int hinfo = INT_MAX;
+    e = hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return st_of(e);
+    *info = (hinfo == INT_MAX) ? 0 : hinfo;
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* spftrs                                                                    */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+struct Solver {
+    curfpHandle_t h;
+    const T *a;
+    T *b;
+    T *inv;
+    int64_t leaf, nrhs;
+
+    hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t K,
+                    double alpha, double beta, int inplace)
+    {
+        GemmArgs g = gemm_args();
+        g.a = x; g.va = vx;
+        g.b = y; g.vb = vy;
+        g.c = b; g.vc = c;
+        g.M = M; g.N = nrhs; g.K = K;
+        g.alpha = alpha; g.beta = beta;
+        g.inplace = inplace;
+        return run_gemm<T>(h, g, RFP_MATH_F32);
+    }
+
+    /* L Y = B in place, L lower (m x m), B (m x nrhs) */
+    hipError_t lower(RfpView L, int64_t m, RfpView B, int64_t slot)
+    {
+        if (m <= 0) return hipSuccess;
+        if (m <= leaf)
+            return gemm(B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, b, B.t(), m, m, 1.0, 0.0, 2);
+        const int64_t m1 = leaf_split(m, leaf);
+        RFP_TRY(lower(L, m1, B, slot));
+        RFP_TRY(gemm(B.sub(m1, 0), a, L.sub(m1, 0), b, B.t(), m - m1, m1, -1.0, 1.0, 0));
+        return lower(L.sub(m1, m1), m - m1, B.sub(m1, 0), slot + m1 / leaf);
+    }
+
+    /* L^T X = B in place */
+    hipError_t lower_t(RfpView L, int64_t m, RfpView B, int64_t slot)
+    {
+        if (m <= 0) return hipSuccess;
+        if (m <= leaf)
+            return gemm(B, inv + slot * leaf * leaf, RfpView{0, 1, leaf}, b, B.t(), m, m, 1.0, 0.0, 2);
+        const int64_t m1 = leaf_split(m, leaf);
+        RFP_TRY(lower_t(L.sub(m1, m1), m - m1, B.sub(m1, 0), slot + m1 / leaf));
+        RFP_TRY(gemm(B, a, L.sub(m1, 0).t(), b, B.sub(m1, 0).t(), m1, m - m1, -1.0, 1.0, 0));
+        return lower_t(L, m1, B, slot);
+    }
+};
+
+template <typename T>
+static curfpStatus_t pftrs_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                int nrhs, const T *A, T *B, int ldb, bool row_major)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || nrhs < 0 || ldb < 1) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
+    if (!A || !B || ldb < (row_major ? nrhs : n)) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const int64_t leaf = rfp_leaf(Prec<T>::f64);
+    const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, (size_t)(s1 + s2) * leaf * leaf * sizeof(T), &ws);
+    if (e != hipSuccess) return st_of(e);
+    T *inv = (T *)ws;
+    e = timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
+              [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); });
+    if (e == hipSuccess)
+        e = timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
+                  [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); });
+    if (e != hipSuccess) return st_of(e);
+
+    Solver<T> sv{h, A, B, inv, leaf, nrhs};
+    /* B(i, c): column-major n x nrhs (the reference ABI) or row-major */
+    const RfpView bv = row_major ? RfpView{0, ldb, 1} : RfpView{0, 1, ldb};
+    const RfpView b2 = bv.sub(g.n1, 0);
+    /* forward: L y = b */
+    e = sv.lower(g.v11, g.n1, bv, 0);
+    if (e == hipSuccess && g.n1 > 0 && g.n2 > 0) e = sv.gemm(b2, A, g.v21, B, bv.t(), g.n2, g.n1, -1.0, 1.0, 0);
+    if (e == hipSuccess) e = sv.lower(g.v22, g.n2, b2, s1);
+    /* backward: L^T x = y */
+    if (e == hipSuccess) e = sv.lower_t(g.v22, g.n2, b2, s1);
+    if (e == hipSuccess && g.n1 > 0 && g.n2 > 0) e = sv.gemm(bv, A, g.v21.t(), B, b2.t(), g.n1, g.n2, -1.0, 1.0, 0);
+    if (e == hipSuccess) e = sv.lower_t(g.v11, g.n1, bv, 0);
+    return st_of(e);
+}
+
+/* ------------------------------------------------------------------------ */
+/* pack / unpack / diagonal                                                  */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+static curfpStatus_t trttf_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                const T *A, int lda, T *arf)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || lda < n) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!A || !arf) return CURFP_STATUS_INVALID_VALUE;
+    const bool lower = uplo == CURFP_FILL_MODE_LOWER;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, lower);
+    const double bytes = 2.0 * (double)n * (n + 1) / 2.0 * sizeof(T);
+    return st_of(timed(h, CURFP_KERNEL_PACK, 0.0, bytes, [&] { return Prec<T>::trttf(g, lower, A, lda, arf, h->stream); }));
+}
+
+template <typename T>
+static curfpStatus_t tfttr_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                const T *arf, T *A, int lda)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || lda < n) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!A || !arf) return CURFP_STATUS_INVALID_VALUE;
+    const bool lower = uplo == CURFP_FILL_MODE_LOWER;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, lower);
+    const double bytes = 2.0 * (double)n * (n + 1) / 2.0 * sizeof(T);
+    return st_of(timed(h, CURFP_KERNEL_PACK, 0.0, bytes, [&] { return Prec<T>::tfttr(g, lower, arf, A, lda, h->stream); }));
+}
+
+template <typename T>
+static curfpStatus_t add_diag_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                   const T *value, T *arf)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || !value) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!arf) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const double bytes = 2.0 * (double)n * sizeof(T);
+    return st_of(timed(h, CURFP_KERNEL_DIAG, 0.0, bytes, [&] { return Prec<T>::add_diag(g, *value, arf, h->stream); }));
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 generator residual                                                   */
+/* ------------------------------------------------------------------------ */
+
+__global__ void residual_init_kernel(int64_t n, int64_t nrhs, double lambda, const double *X, int64_t ldx,
+                                     const double *B, int64_t ldb, double *R, int64_t ldr)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t i = idx % n, c = idx / n;
+    R[i + c * ldr] = B[i + c * ldb] - lambda * X[i + c * ldx];
+}
+
+extern "C" curfpStatus_t curfpSgenResidual(curfpHandle_t h, curfpOperation_t trans, int n, int k, int nrhs,
+                                           const double *alpha, const float *A, int lda, const double *lambda,
+                                           const double *X, int ldx, const double *B, int ldb, double *R,
+                                           int ldr)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || k < 0 || nrhs < 0 || lda < 1 || !alpha || !lambda) return CURFP_STATUS_INVALID_VALUE;
+    if (trans != CURFP_OP_N && trans != CURFP_OP_T) return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
+    if (!X || !B || !R || ldx < n || ldb < n || ldr < n) return CURFP_STATUS_INVALID_VALUE;
+    if (k > 0 && !A) return CURFP_STATUS_INVALID_VALUE;
+    /* R = B - lambda X */
+    {
+        const int64_t total = (int64_t)n * nrhs;
+        hipLaunchKernelGGL(residual_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream,
+                           (int64_t)n, (int64_t)nrhs, *lambda, X, (int64_t)ldx, B, (int64_t)ldb, R, (int64_t)ldr);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return st_of(e);
+    }
+    if (k == 0 || *alpha == 0.0) return CURFP_STATUS_SUCCESS;
+    /* T = op(A)^T X (k x nrhs), split-K over n; then R -= alpha op(A) T */
+    const int64_t splits = n >= 8192 ? 32 : 1;
+    const size_t t_elems = (size_t)k * nrhs;
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, (t_elems * (1 + splits)) * sizeof(double), &ws);
+    if (e != hipSuccess) return st_of(e);
+    double *T = (double *)ws;
+    const RfpView av = (trans == CURFP_OP_N) ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
+    GemmArgs g1 = gemm_args();
+    g1.a = A; g1.va = av.t();
+    g1.b = X; g1.vb = RfpView{0, ldx, 1};
+    g1.c = T; g1.vc = RfpView{0, 1, k};
+    g1.M = k; g1.N = nrhs; g1.K = n;
+    g1.alpha = 1.0; g1.beta = 0.0;
+    g1.a_f32 = true;
+    g1.splits = (int)splits;
+    g1.partial = T + t_elems;
+    e = run_gemm<double>(h, g1, RFP_MATH_F64);
+    if (e != hipSuccess) return st_of(e);
+    GemmArgs g2 = gemm_args();
+    g2.a = A; g2.va = av;
+    g2.b = T; g2.vb = RfpView{0, k, 1};
+    g2.c = R; g2.vc = RfpView{0, 1, ldr};
+    g2.M = n; g2.N = nrhs; g2.K = k;
+    g2.alpha = -*alpha; g2.beta = 1.0;
+    g2.a_f32 = true;
+    return st_of(run_gemm<double>(h, g2, RFP_MATH_F64));
+}
+
+/* ------------------------------------------------------------------------ */
+/* C ABI                                                                     */
+/* ------------------------------------------------------------------------ */
+
+extern "C" {
+
+curfpStatus_t curfpSsfrk(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpOperation_t trans,
+                         int n, int k, const float *alpha, const float *A, int lda, const float *beta, float *C)
+{ return sfrk_impl<float>(h, transr, uplo, trans, n, k, alpha, A, lda, beta, C, false); }
+
+curfpStatus_t curfpSsfrkBf16(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo,
+                             curfpOperation_t trans, int n, int k, const float *alpha, const uint16_t *A,
+                             int lda, const float *beta, float *C)
+{ return sfrk_impl<float>(h, transr, uplo, trans, n, k, alpha, A, lda, beta, C, true); }
+
+curfpStatus_t curfpDsfrk(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpOperation_t trans,
+                         int n, int k, const double *alpha, const double *A, int lda, const double *beta,
+                         double *C)
+{ return sfrk_impl<double>(h, transr, uplo, trans, n, k, alpha, A, lda, beta, C, false); }
+
+curfpStatus_t curfpSpftrf(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, float *A,
+                          int *info)
+{ return pftrf_impl<float>(h, transr, uplo, n, A, info); }
+
+curfpStatus_t curfpDpftrf(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, double *A,
+                          int *info)
+{ return pftrf_impl<double>(h, transr, uplo, n, A, info); }
+
+curfpStatus_t curfpSpftrs(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, int nrhs,
+                          const float *A, float *B, int ldb)
+{ return pftrs_impl<float>(h, transr, uplo, n, nrhs, A, B, ldb, false); }
+
+curfpStatus_t curfpSpftrsRowMajor(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                  int nrhs, const float *A, float *B, int ldb)
+{ return pftrs_impl<float>(h, transr, uplo, n, nrhs, A, B, ldb, true); }
+
+curfpStatus_t curfpDpftrs(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, int nrhs,
+                          const double *A, double *B, int ldb)
+{ return pftrs_impl<double>(h, transr, uplo, n, nrhs, A, B, ldb, false); }
+
+curfpStatus_t curfpDpftrsRowMajor(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                  int nrhs, const double *A, double *B, int ldb)
+{ return pftrs_impl<double>(h, transr, uplo, n, nrhs, A, B, ldb, true); }
+
+curfpStatus_t curfpSstrttf(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                           const float *A, int lda, float *arf)
+{ return trttf_impl<float>(h, transr, uplo, n, A, lda, arf); }
+
+curfpStatus_t curfpDstrttf(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                           const double *A, int lda, double *arf)
+{ return trttf_impl<double>(h, transr, uplo, n, A, lda, arf); }
+
+curfpStatus_t curfpSstfttr(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                           const float *arf, float *A, int lda)
+{ return tfttr_impl<float>(h, transr, uplo, n, arf, A, lda); }
+
+curfpStatus_t curfpDstfttr(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                           const double *arf, double *A, int lda)
+{ return tfttr_impl<double>(h, transr, uplo, n, arf, A, lda); }
+
+curfpStatus_t curfpSaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                  const float *value, float *arf)
+{ return add_diag_impl<float>(h, transr, uplo, n, value, arf); }
+
+curfpStatus_t curfpDaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                  const double *value, double *arf)
+{ return add_diag_impl<double>(h, transr, uplo, n, value, arf); }
+
+}  /* extern "C" */

@@ -1,0 +1,203 @@
+/*
+ * rfp_internal.h — handle layout, RFP geometry and launcher declarations
+ * shared by the host drivers and the HIP kernels.
+ *
+ * RFP geometry as three strided views
+ * -----------------------------------
+ * LAPACK's RFP format (the reference's sub-block tables,
+ * src/curfp_ssfrk.cpp:52-133, src/curfp_spftrf.cpp:62-180,
+ * src/curfp_spftrs.cpp:60-213) splits the n x n symmetric matrix into a
+ * leading n1 x n1 block, a trailing n2 x n2 block and the n2 x n1
+ * off-diagonal block.  Here every one of the 8 storage variants is described
+ * by the LOWER-triangular logical factorisation C = L*L^T (for uplo = U,
+ * L = U^T), with each block of L (and of the unfactored C's lower triangle)
+ * reached through a strided view
+ *
+ *     element (i, j) of the block  ->  arf[off + i*rs + j*cs]
+ *
+ * where (rs, cs) is (1, ld) ("column view") or (ld, 1) ("row view").  The
+ * three views are
+ *     v11: n1 x n1 lower triangle   (row view iff transr = T)
+ *     v21: n2 x n1 rectangle        (column view iff (transr = N) == (uplo = L))
+ *     v22: n2 x n2 lower triangle   (row view iff transr = N)
+ * and the offsets/ld are LAPACK's.  This one description replaces the
+ * reference's per-routine case tables: ssfrk writes the three blocks,
+ * spftrf factors v11, solves v21, updates and factors v22, and spftrs runs
+ * the forward and backward block substitutions on them.  The mapping is
+ * checked element by element against LAPACK ?trttf for all 8 variants.
+ */
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <vector>
+#include "../../include/curfp.h"
+
+struct RfpView {
+    int64_t off, rs, cs;
+    __host__ __device__ int64_t at(int64_t i, int64_t j) const { return off + i * rs + j * cs; }
+    /* sub-view starting at block (i0, j0) */
+    __host__ __device__ RfpView sub(int64_t i0, int64_t j0) const { return RfpView{at(i0, j0), rs, cs}; }
+    /* transposed view: (i, j) of the result is (j, i) of this one */
+    __host__ __device__ RfpView t() const { return RfpView{off, cs, rs}; }
+};
+
+struct RfpGeom {
+    int64_t n, n1, n2, ld;
+    RfpView v11, v21, v22;
+};
+
+/* LAPACK's n1/n2 split and sub-block offsets, expressed as the three views. */
+static inline RfpGeom rfp_geom(int64_t n, bool transr_n, bool lower)
+{
+    RfpGeom g;
+    g.n = n;
+    int64_t o11, o21, o22;
+    if (n % 2) {
+        if (lower) { g.n1 = n - n / 2; g.n2 = n / 2; }
+        else       { g.n1 = n / 2;     g.n2 = n - n / 2; }
+        const int64_t n1 = g.n1, n2 = g.n2;
+        if (transr_n) {
+            g.ld = n;
+            if (lower) { o11 = 0;  o21 = n1; o22 = n;  }
+            else       { o11 = n2; o21 = 0;  o22 = n1; }
+        } else if (lower) {
+            g.ld = n1; o11 = 0; o21 = n1 * n1; o22 = 1;
+        } else {
+            g.ld = n2; o11 = n2 * n2; o21 = 0; o22 = n1 * n2;
+        }
+    } else {
+        const int64_t k = n / 2;
+        g.n1 = g.n2 = k;
+        if (transr_n) {
+            g.ld = n + 1;
+            if (lower) { o11 = 1;     o21 = k + 1; o22 = 0; }
+            else       { o11 = k + 1; o21 = 0;     o22 = k; }
+        } else {
+            g.ld = k;
+            if (lower) { o11 = k;           o21 = k * (k + 1); o22 = 0;     }
+            else       { o11 = k * (k + 1); o21 = 0;           o22 = k * k; }
+        }
+    }
+    const bool row11 = !transr_n;
+    const bool row22 = transr_n;
+    const bool row21 = !(transr_n == lower);
+    auto mk = [&](int64_t o, bool row) {
+        return row ? RfpView{o, g.ld, 1} : RfpView{o, 1, g.ld};
+    };
+    g.v11 = mk(o11, row11);
+    g.v21 = mk(o21, row21);
+    g.v22 = mk(o22, row22);
+    return g;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Handle                                                                    */
+/* ------------------------------------------------------------------------ */
+/* One profiled launch: a pair of events on the handle stream around it. */
+struct RfpProfRecord {
+    int cls;
+    double flops, bytes;
+    hipEvent_t e0, e1;
+};
+
+struct RfpKernelStat {
+    int64_t launches;
+    double flops, bytes, ms;
+};
+
+struct curfpContext {
+    hipStream_t     stream;
+    int             device;
+    curfpMathMode_t math;
+    void           *ws;        /* cached device workspace (grown on demand) */
+    size_t          ws_bytes;
+    int            *d_info;    /* device int: first failing minor (INT_MAX = none) */
+    /* kernel profiling (curfpSetProfiling / curfpGetKernelStats) */
+    bool            prof;
+    std::vector<RfpProfRecord> prof_pending;
+    std::vector<hipEvent_t>    prof_pool;
+    RfpKernelStat   prof_stats[CURFP_KERNEL_CLASSES];
+};
+
+/* Returns a device workspace of at least `bytes` bytes owned by the handle. */
+hipError_t rfp_workspace(curfpHandle_t h, size_t bytes, void **out);
+
+/* ------------------------------------------------------------------------ */
+/* Kernel launchers (implemented in the .hip files)                          */
+/* ------------------------------------------------------------------------ */
+
+/* Compute precision of a view GEMM. */
+enum RfpMath { RFP_MATH_F32 = 0, RFP_MATH_BF16 = 1, RFP_MATH_BF16X3 = 2, RFP_MATH_F64 = 3 };
+
+/*
+ * C(i, j) = alpha * sum_k A(i, k) * B(j, k) + beta * C(i, j)
+ * for 0 <= i < M, 0 <= j < N (and only i >= j when `lower`, which requires
+ * M == N and a square diagonal block).  A, B, C are strided views over the
+ * given base pointers; each view must have a unit stride in one direction.
+ * beta == 0 never reads C.  `inplace` = 1 / 2 declares that C aliases A
+ * (resp. B); the kernel then gives one workgroup every column (resp. row)
+ * of its output rows (columns), and the caller keeps N (resp. M) within one
+ * tile (gemm_inplace_limit).
+ */
+struct GemmArgs {
+    const void *a; RfpView va;
+    const void *b; RfpView vb;
+    void *c;       RfpView vc;
+    int64_t M, N, K;
+    double alpha, beta;
+    bool lower;
+    int  inplace;      /* 0 none, 1: C aliases A (one WG spans all N), 2: C aliases B (one WG spans all M) */
+    bool a_bf16;       /* A and B are raw BF16 storage (ssfrk with BF16 input) */
+    bool a_f32;        /* FP64 GEMM whose A is stored as FP32 (generator residual) */
+    int  splits;       /* split-K slices (needs `partial`: splits*M*N elements) */
+    void *partial;
+};
+static inline GemmArgs gemm_args() { GemmArgs g{}; g.splits = 1; return g; }
+hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s);
+hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s);
+
+/* Largest N (inplace = 1) / M (inplace = 2) the in-place GEMM supports. */
+int gemm_inplace_limit(bool f64);
+
+/* Leaf size of the recursive factorisation / solves. */
+int rfp_leaf(bool f64);
+
+/*
+ * Diagonal tile Cholesky, in place on view v (m x m lower, m <= leaf).
+ * On failure at local column j, atomicMin(info, info_base + j + 1).
+ * When inv != nullptr, also writes inv(L) as a dense leaf x leaf row-major
+ * lower-triangular matrix.  Skips all work if *info already holds a value
+ * smaller than INT_MAX (an earlier block failed).
+ */
+hipError_t launch_potrf_tile_f32(float *base, RfpView v, int m, int *info, int info_base,
+                                 float *inv, hipStream_t s);
+hipError_t launch_potrf_tile_f64(double *base, RfpView v, int m, int *info, int info_base,
+                                 double *inv, hipStream_t s);
+
+/*
+ * Batched inverse of lower-triangular diagonal tiles: tile t covers rows
+ * [t*leaf, min(m, (t+1)*leaf)) of the lower-triangular view v (m x m);
+ * inv(tile t) goes to inv + t*leaf*leaf (row-major, zero upper part).
+ */
+hipError_t launch_trtri_tiles_f32(const float *base, RfpView v, int64_t m, float *inv, hipStream_t s);
+hipError_t launch_trtri_tiles_f64(const double *base, RfpView v, int64_t m, double *inv, hipStream_t s);
+
+/* RFP <-> dense row-major triangle copies (all 8 variants). */
+hipError_t launch_trttf_f32(const RfpGeom &g, bool lower, const float *A, int64_t lda, float *arf, hipStream_t s);
+hipError_t launch_trttf_f64(const RfpGeom &g, bool lower, const double *A, int64_t lda, double *arf, hipStream_t s);
+hipError_t launch_tfttr_f32(const RfpGeom &g, bool lower, const float *arf, float *A, int64_t lda, hipStream_t s);
+hipError_t launch_tfttr_f64(const RfpGeom &g, bool lower, const double *arf, double *A, int64_t lda, hipStream_t s);
+
+/* arf[diag] += value; fill arf[0, nt) with zero. */
+hipError_t launch_add_diag_f32(const RfpGeom &g, float value, float *arf, hipStream_t s);
+hipError_t launch_add_diag_f64(const RfpGeom &g, double value, double *arf, hipStream_t s);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+hipError_t launch_set_int(int *p, int value, hipStream_t s);
+
+/* FP64 generator residual, see curfpSgenResidual. */
+hipError_t launch_gen_residual(bool trans_n, int64_t n, int64_t k, int64_t nrhs, double alpha,
+                               const float *A, int64_t lda, double lambda,
+                               const double *X, int64_t ldx, const double *B, int64_t ldb,
+                               double *R, int64_t ldr, double *tmp, hipStream_t s);

@@ -1,0 +1,230 @@
+/*
+ * curfp.h — C ABI of the MI355X (gfx950) Rectangular Full Packed (RFP) library.
+ *
+ * Drop-in for the hot path of the reference header include/curfp.h:
+ * same entry-point names, same enum values, same argument meaning, same
+ * status codes.  The only type change is cudaStream_t -> hipStream_t; the
+ * cuBLAS / cuSOLVER includes are gone because nothing here calls them.
+ *
+ * Conventions (identical to the reference):
+ *   - every matrix is a caller-owned device pointer;
+ *   - alpha, beta, value and info are HOST pointers;
+ *   - every operation is enqueued on the stream bound to the handle
+ *     (curfpSetStream); the default is the null stream;
+ *   - spftrf is host-synchronous (it returns info through a host pointer),
+ *     everything else is asynchronous on the handle's stream;
+ *   - no entry point throws; all return a curfpStatus_t.
+ *
+ * All internal indexing is 64-bit, so n up to 2^31-1 is accepted as long as
+ * the n*(n+1)/2 packed array fits in device memory (n = 65536 already has
+ * n*(n+1)/2 > INT_MAX).
+ */
+#ifndef CURFP_H
+#define CURFP_H
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes — reference include/curfp.h:14-20 (same values). */
+typedef enum {
+    CURFP_STATUS_SUCCESS          = 0,
+    CURFP_STATUS_NOT_INITIALIZED  = 1,
+    CURFP_STATUS_ALLOC_FAILED     = 2,
+    CURFP_STATUS_INVALID_VALUE    = 3,
+    CURFP_STATUS_EXECUTION_FAILED = 4
+} curfpStatus_t;
+
+/* TRANSR / TRANS — reference include/curfp.h:25-28. */
+typedef enum {
+    CURFP_OP_N = 0,
+    CURFP_OP_T = 1
+} curfpOperation_t;
+
+/* UPLO — reference include/curfp.h:33-36. */
+typedef enum {
+    CURFP_FILL_MODE_LOWER = 0,
+    CURFP_FILL_MODE_UPPER = 1
+} curfpFillMode_t;
+
+/*
+ * Arithmetic used by the contractions of ssfrk and of the spftrf updates
+ * (TRSM / SYRK / GEMM).  No counterpart in the reference (which always runs
+ * cuBLAS FP32); the FP64 d* routines ignore it.
+ *   CURFP_MATH_FP32   : FP32 inputs, v_mfma_f32_32x32x2_f32, FP32 accumulate (default).
+ *   CURFP_MATH_BF16   : operands rounded to BF16, v_mfma_f32_32x32x16_bf16, FP32 accumulate.
+ *   CURFP_MATH_BF16X3 : split-BF16 (x = hi + lo, hi*hi + hi*lo + lo*hi), FP32 accumulate.
+ * The small diagonal-tile Cholesky and triangular inverses always run in FP32.
+ */
+typedef enum {
+    CURFP_MATH_FP32   = 0,
+    CURFP_MATH_BF16   = 1,
+    CURFP_MATH_BF16X3 = 2
+} curfpMathMode_t;
+
+typedef struct curfpContext *curfpHandle_t;
+
+/* Handle management — reference include/curfp.h:63-66, src/curfp_handle.cpp:4-51. */
+curfpStatus_t curfpCreate(curfpHandle_t *handle);
+curfpStatus_t curfpDestroy(curfpHandle_t handle);
+curfpStatus_t curfpSetStream(curfpHandle_t handle, hipStream_t stream);
+curfpStatus_t curfpGetStream(curfpHandle_t handle, hipStream_t *stream);
+
+/* Math mode for the S routines (no reference counterpart). */
+curfpStatus_t curfpSetMathMode(curfpHandle_t handle, curfpMathMode_t mode);
+curfpStatus_t curfpGetMathMode(curfpHandle_t handle, curfpMathMode_t *mode);
+
+/*
+ * curfpSsfrk — reference include/curfp.h:79-91, src/curfp_ssfrk.cpp:135-212.
+ *   trans = N: C := alpha*A*A^T + beta*C, A is n x k column-major (lda >= n)
+ *   trans = T: C := alpha*A^T*A + beta*C, A is k x n column-major (lda >= k),
+ *              i.e. a row-major (n, k) array.
+ * C is the n x n symmetric matrix in RFP format (n*(n+1)/2 elements).
+ * Quick returns as in the reference: n == 0; (alpha == 0 or k == 0) and
+ * beta == 1 is a no-op; alpha == beta == 0 zero-fills C (A is never read).
+ */
+curfpStatus_t curfpSsfrk(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, curfpOperation_t trans,
+                         int n, int k, const float *alpha, const float *A,
+                         int lda, const float *beta, float *C);
+
+/* curfpSpftrf — reference include/curfp.h:105-112, src/curfp_spftrf.cpp:182-292.
+ * In-place Cholesky of the RFP matrix (uplo = L: A = L*L^T, uplo = U: A = U^T*U).
+ * *info = 0 on success, > 0 = order of the first leading minor that is not
+ * positive definite (the factor is then partially overwritten). */
+curfpStatus_t curfpSpftrf(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, float *A, int *info);
+
+/* curfpSpftrs — reference include/curfp.h:123-132, src/curfp_spftrs.cpp:215-302.
+ * Solves A*X = B with the RFP factor from curfpSpftrf; B is n x nrhs
+ * column-major (ldb >= n) and is overwritten with X. */
+curfpStatus_t curfpSpftrs(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, int nrhs,
+                          const float *A, float *B, int ldb);
+
+/* curfpSpftrsRowMajor — curfpSpftrs with B stored ROW-major (n x nrhs,
+ * ldb >= nrhs), the layout of the reference's Python API
+ * (curfp/__init__.py:540), which otherwise transposes B into a column-major
+ * copy and back (curfp/__init__.py:548, :562).  No reference C counterpart. */
+curfpStatus_t curfpSpftrsRowMajor(curfpHandle_t handle, curfpOperation_t transr,
+                                  curfpFillMode_t uplo, int n, int nrhs,
+                                  const float *A, float *B, int ldb);
+
+/* curfpSstrttf — reference include/curfp.h:141-149, src/curfp_strttf.cu:421-462.
+ * Dense n x n ROW-MAJOR triangle (lda >= n) -> RFP. */
+curfpStatus_t curfpSstrttf(curfpHandle_t handle, curfpOperation_t transr,
+                           curfpFillMode_t uplo, int n, const float *A,
+                           int lda, float *arf);
+
+/* curfpSstfttr — reference include/curfp.h:197-205, src/curfp_stfttr.cu:134-175.
+ * RFP -> dense n x n ROW-MAJOR triangle (lda >= n); the other triangle is
+ * not written. */
+curfpStatus_t curfpSstfttr(curfpHandle_t handle, curfpOperation_t transr,
+                           curfpFillMode_t uplo, int n, const float *arf,
+                           float *A, int lda);
+
+/* curfpSaddToDiagonal — C entry point for the reference's pure-torch
+ * add_to_diagonal (curfp/__init__.py:194-296): arf[diag] += *value for the n
+ * diagonal positions of the RFP matrix, one kernel, on the handle stream.
+ * The result is fl(arf + value) elementwise. */
+curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
+                                  curfpFillMode_t uplo, int n,
+                                  const float *value, float *arf);
+
+/* curfpSsfrkBf16 — ssfrk with a BF16 A (raw 16-bit storage, same layout and
+ * lda meaning as curfpSsfrk).  Products of BF16 values are exact in FP32, so
+ * the result is the FP32-accumulated A*A^T of the BF16 matrix.  No reference
+ * counterpart. */
+curfpStatus_t curfpSsfrkBf16(curfpHandle_t handle, curfpOperation_t transr,
+                             curfpFillMode_t uplo, curfpOperation_t trans,
+                             int n, int k, const float *alpha,
+                             const uint16_t *A, int lda, const float *beta,
+                             float *C);
+
+/*
+ * curfpSgenResidual — FP64 residual for iterative refinement (no reference
+ * counterpart):
+ *   R := B - (alpha * op(A) * op(A)^T + lambda * I) * X        (all n x nrhs)
+ * computed from the generator A (FP32, trans/lda as in curfpSsfrk) with FP64
+ * products and FP64 accumulation.  X, B, R are FP64, column-major, with
+ * leading dimensions ldx, ldb, ldr.  R may alias B.
+ */
+curfpStatus_t curfpSgenResidual(curfpHandle_t handle, curfpOperation_t trans,
+                                int n, int k, int nrhs, const double *alpha,
+                                const float *A, int lda, const double *lambda,
+                                const double *X, int ldx, const double *B,
+                                int ldb, double *R, int ldr);
+
+/*
+ * Kernel profiling (no reference counterpart; the reference has only
+ * CUDA-event timing in its benchmark scripts).  When enabled, every kernel
+ * launch of the library is bracketed by a pair of HIP events on the handle
+ * stream and tagged with its algorithmic work (flops, or bytes for the
+ * bandwidth kernels).  curfpGetKernelStats synchronises the stream, folds
+ * the finished launches into per-class totals and returns them.
+ */
+typedef enum {
+    CURFP_KERNEL_GEMM_F32   = 0,   /* view GEMM, FP32 MFMA                */
+    CURFP_KERNEL_GEMM_BF16  = 1,   /* view GEMM, BF16 / split-BF16 MFMA   */
+    CURFP_KERNEL_GEMM_F64   = 2,   /* view GEMM, FP64 MFMA                */
+    CURFP_KERNEL_POTRF_TILE = 3,   /* leaf Cholesky + inverse             */
+    CURFP_KERNEL_TRTRI      = 4,   /* batched leaf inverses (spftrs)      */
+    CURFP_KERNEL_PACK       = 5,   /* strttf / stfttr                     */
+    CURFP_KERNEL_DIAG       = 6,   /* add_to_diagonal                     */
+    CURFP_KERNEL_OTHER      = 7,   /* fills, reductions, residual setup   */
+    CURFP_KERNEL_CLASSES    = 8
+} curfpKernelClass_t;
+
+curfpStatus_t curfpSetProfiling(curfpHandle_t handle, int enable);
+/* Totals since the last reset for one class; reset != 0 clears them after reading. */
+curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelClass_t cls,
+                                  int64_t *launches, double *flops,
+                                  double *bytes, double *milliseconds, int reset);
+
+/* ---------------------------- FP64 twins -------------------------------- */
+
+/* curfpDsfrk  — reference include/curfp.h:357-369, src/curfp_dssfrk.cpp:101-172. */
+curfpStatus_t curfpDsfrk(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, curfpOperation_t trans,
+                         int n, int k, const double *alpha, const double *A,
+                         int lda, const double *beta, double *C);
+
+/* curfpDpftrf — reference include/curfp.h:374-381, src/curfp_dpftrf.cpp:140-244. */
+curfpStatus_t curfpDpftrf(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, double *A, int *info);
+
+/* curfpDpftrs — reference include/curfp.h:386-395, src/curfp_dpftrs.cpp:117-195. */
+curfpStatus_t curfpDpftrs(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, int nrhs,
+                          const double *A, double *B, int ldb);
+
+/* curfpDpftrsRowMajor — FP64 twin of curfpSpftrsRowMajor
+ * (reference Python layout curfp/__init__.py:1547-1581). */
+curfpStatus_t curfpDpftrsRowMajor(curfpHandle_t handle, curfpOperation_t transr,
+                                  curfpFillMode_t uplo, int n, int nrhs,
+                                  const double *A, double *B, int ldb);
+
+/* curfpDstrttf — reference include/curfp.h:400-408, src/curfp_dstrttf.cu:111-152. */
+curfpStatus_t curfpDstrttf(curfpHandle_t handle, curfpOperation_t transr,
+                           curfpFillMode_t uplo, int n, const double *A,
+                           int lda, double *arf);
+
+/* curfpDstfttr — reference include/curfp.h:441-449, src/curfp_dstfttr.cu:111-152. */
+curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
+                           curfpFillMode_t uplo, int n, const double *arf,
+                           double *A, int lda);
+
+/* curfpDaddToDiagonal — FP64 twin of curfpSaddToDiagonal
+ * (reference curfp/__init__.py:272-296 accepts float64 too). */
+curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
+                                  curfpFillMode_t uplo, int n,
+                                  const double *value, double *arf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CURFP_H */

@@ -1,0 +1,198 @@
+"""CPU tests: pin the oracle against LAPACK golden vectors, the reference's
+hand-written known answers and mathematical invariants.  No GPU needed."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import rfp_oracle as O
+
+VARIANTS = [(t, u) for t in "NT" for u in "LU"]
+N_PERM = [1, 2, 3, 4, 5, 6, 7, 8, 15, 16, 17, 32, 33, 63, 64, 65, 128, 129, 257]
+
+
+# --------------------------------------------------------------------------- geometry
+
+@pytest.mark.parametrize("n", N_PERM)
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_index_map_matches_lapack_trttf(golden, n, t, u):
+    """strttf of A[i, j] = i*n + j + 1 through LAPACK is the index permutation."""
+    perm = golden("rfp_perm.npz")[f"n{n}_{t}{u}"].astype(np.int64)
+    A = np.arange(1, n * n + 1, dtype=np.int64).reshape(n, n)
+    assert np.array_equal(O.trttf(A, t, u), perm)
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 6, 17, 64, 65, 129])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_index_map_is_bijection_and_round_trips(n, t, u):
+    M = O.lower_index_map(n, t, u)
+    i, j = np.tril_indices(n)
+    idx = M[i, j]
+    assert np.array_equal(np.sort(idx), np.arange(n * (n + 1) // 2))
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    arf = O.trttf(A, t, u)
+    back = O.tfttr(arf, n, t, u)
+    tri = np.tril(A) if u == "L" else np.triu(A)
+    assert np.array_equal(back, tri)
+
+
+def test_hand_layouts_from_reference_tests():
+    # n=2, TRANSR=N, UPLO=L (case 5): ARF = {A(1,1), A(0,0), A(1,0)}
+    # (python_tests/test_spftrf.py:4-6, python_tests/test_ssfrk.py:84-111)
+    M = O.lower_index_map(2, "N", "L")
+    assert (M[1, 1], M[0, 0], M[1, 0]) == (0, 1, 2)
+    # n=3, TRANSR=N, UPLO=L (case 1): {A00, A10, A20, A22, A11, A21}
+    # (python_tests/test_spftrf.py:8-11)
+    M = O.lower_index_map(3, "N", "L")
+    assert [M[0, 0], M[1, 0], M[2, 0], M[2, 2], M[1, 1], M[2, 1]] == [0, 1, 2, 3, 4, 5]
+    # LAPACK documentation example, n=6, TRANSR=T, UPLO=U (rows of the k x (n+1) array):
+    #   03 13 23 33 00 01 02 / 04 14 24 34 44 11 12 / 05 15 25 35 45 55 22
+    rows = [["03", "13", "23", "33", "00", "01", "02"],
+            ["04", "14", "24", "34", "44", "11", "12"],
+            ["05", "15", "25", "35", "45", "55", "22"]]
+    A = np.arange(36).reshape(6, 6)
+    arf = O.trttf(A, "T", "U")
+    R = arf.reshape(7, 3).T  # column-major k x (n+1)
+    for r in range(3):
+        for c in range(7):
+            i, j = int(rows[r][c][0]), int(rows[r][c][1])
+            assert R[r, c] == A[i, j]
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 7, 64, 65])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_diag_indices_closed_form(n, t, u):
+    M = O.lower_index_map(n, t, u)
+    assert np.array_equal(O.diag_indices(n, t, u), M[np.arange(n), np.arange(n)])
+
+
+def test_planted_geometry_mistake_is_caught(golden, monkeypatch):
+    """A plausible slip (v21 read as a column view instead of a row view, i.e.
+    a transposed off-diagonal block) must fail the golden comparison."""
+    real = O.views
+
+    def wrong(n, transr="T", uplo="U"):
+        n1, n2, ld, v11, v21, v22 = real(n, transr, uplo)
+        return n1, n2, ld, v11, (v21[0], v21[2], v21[1]), v22
+
+    monkeypatch.setattr(O, "views", wrong)
+    perm = golden("rfp_perm.npz")["n8_TU"].astype(np.int64)
+    A = np.arange(1, 65, dtype=np.int64).reshape(8, 8)
+    assert not np.array_equal(O.trttf(A, "T", "U"), perm)
+
+
+# --------------------------------------------------------------------------- arithmetic
+
+CHAIN = [(1, 1), (2, 2), (3, 3), (4, 4), (5, 5), (8, 6), (9, 7), (32, 16), (33, 17),
+         (64, 32), (65, 33), (128, 64), (129, 65), (256, 128), (257, 129)]
+
+
+@pytest.mark.parametrize("n,k", CHAIN)
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_oracle_chain_matches_lapack_f32(golden, n, k, t, u):
+    g = golden("chain_f32.npz")
+    key = f"n{n}_k{k}_{t}{u}"
+    A, Cl, Fl, B, Xl = (g[key + s] for s in ("_A", "_C", "_F", "_B", "_X"))
+    C = O.sfrk(n, k, 1.0, A, 0.0, np.zeros_like(Cl), t, u, "N")
+    C = O.add_to_diagonal(C, 1.0, t, u)
+    # the reference's own bar is 1e-3 (python_tests/test_vs_lapack.py:22-23)
+    assert np.abs(C - Cl).max() <= 1e-5 * max(1.0, np.abs(Cl).max())
+    F, info = O.pftrf(n, C, t, u)
+    assert info == 0
+    assert np.abs(F - Fl).max() <= 1e-4
+    X = O.pftrs(n, F, B, t, u)
+    assert np.abs(X - Xl).max() <= 1e-4 * max(1.0, np.abs(Xl).max())
+
+
+@pytest.mark.parametrize("n,k", [(1, 1), (2, 2), (5, 5), (9, 7), (33, 17), (64, 32), (129, 65)])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_oracle_chain_matches_lapack_f64(golden, n, k, t, u):
+    g = golden("chain_f64.npz")
+    key = f"n{n}_k{k}_{t}{u}"
+    A, Cl, Fl, B, Xl = (g[key + s] for s in ("_A", "_C", "_F", "_B", "_X"))
+    C = O.add_to_diagonal(O.sfrk(n, k, 1.0, A, 0.0, np.zeros_like(Cl), t, u, "N"), 1.0, t, u)
+    assert np.abs(C - Cl).max() <= 1e-13
+    F, info = O.pftrf(n, C, t, u)
+    assert info == 0 and np.abs(F - Fl).max() <= 1e-12
+    X = O.pftrs(n, F, B, t, u)
+    assert np.abs(X - Xl).max() <= 1e-11
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("n,k", [(7, 3), (16, 5), (33, 9)])
+def test_sfrk_exact_small_integers(t, u, n, k):
+    """Small-integer A: every product and partial sum is exact in float32, so
+    the packed result must equal trttf(A A^T) bit for bit."""
+    rng = np.random.default_rng(n * 31 + k)
+    A = rng.integers(-4, 5, size=(n, k)).astype(np.float32)
+    C = O.sfrk(n, k, 1.0, A, 0.0, np.zeros(n * (n + 1) // 2, np.float32), t, u, "N")
+    S = (A.astype(np.int64) @ A.T.astype(np.int64)).astype(np.float32)
+    assert np.array_equal(C, O.dense_sym_to_rfp(S, t, u))
+
+
+def test_sfrk_quick_returns():
+    """python_tests/test_ssfrk.py:12-56: alpha=0, beta=1 passes C through;
+    alpha=beta=0 zeroes C; neither reads A."""
+    C = np.arange(15, dtype=np.float32)
+    assert np.array_equal(O.sfrk(5, 3, 0.0, None, 1.0, C, "T", "U", "N"), C)
+    assert np.array_equal(O.sfrk(5, 3, 0.0, None, 0.0, C, "T", "U", "N"), np.zeros(15, np.float32))
+
+
+def test_sfrk_n2_rank1_known_answer():
+    """python_tests/test_ssfrk.py:84-111 (case 5, alpha=2, beta=0.5)."""
+    C = np.array([4.0, 1.0, 0.5], np.float32)  # {C11, C00, C10}
+    A = np.array([[2.0], [3.0]], np.float32)
+    out = O.sfrk(2, 1, 2.0, A, 0.5, C, "N", "L", "N")
+    assert np.allclose(out, [2 * 9 + 2.0, 2 * 4 + 0.5, 2 * 6 + 0.25])
+
+
+def test_pftrf_known_answers():
+    """python_tests/test_spftrf.py:19-128."""
+    for t, u in VARIANTS:
+        F, info = O.pftrf(1, np.array([9.0], np.float32), t, u)
+        assert info == 0 and abs(F[0] - 3.0) < 1e-6
+    F, info = O.pftrf(2, np.array([3.0, 4.0, 2.0], np.float32), "N", "L")
+    assert info == 0 and np.allclose(F, [math.sqrt(2.0), 2.0, 1.0], atol=1e-6)
+    A = np.array([5.0, 1.0, 2.0, 6.0, 4.0, 0.0], np.float32)
+    F, info = O.pftrf(3, A, "N", "L")
+    l00 = math.sqrt(5)
+    l10, l20 = 1 / l00, 2 / l00
+    l11 = math.sqrt(4 - l10 ** 2)
+    l21 = (0 - l20 * l10) / l11
+    l22 = math.sqrt(6 - l20 ** 2 - l21 ** 2)
+    assert info == 0 and np.allclose(F, [l00, l10, l20, l22, l11, l21], atol=1e-6)
+
+
+def test_pftrf_info_matches_lapack(golden):
+    g = golden("info.npz")
+    keys = sorted(k[:-5] for k in g.files if k.endswith("_info"))
+    for key in keys:
+        arf = g[key + "_arf"]
+        n = int(key.split("_")[1][1:])
+        t, u = key[-2], key[-1]
+        _, info = O.pftrf(n, arf, t, u)
+        assert info == int(g[key + "_info"]), key
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_pftrf_reconstruction_and_residual(t, u):
+    n, k = 97, 40
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((n, k))
+    C = O.add_to_diagonal(O.sfrk(n, k, 1.0, A, 0.0, np.zeros(n * (n + 1) // 2), t, u, "N"), 0.5, t, u)
+    S = O.rfp_to_dense_sym(C, n, t, u)
+    F, info = O.pftrf(n, C, t, u)
+    L = O.rfp_lower_factor(F, n, t, u)
+    assert info == 0
+    assert np.linalg.norm(L @ L.T - S) / np.linalg.norm(S) < 1e-14
+    B = rng.standard_normal((n, 4))
+    X = O.pftrs(n, F, B, t, u)
+    assert O.dense_relative_residual(S, X, B) < 1e-15
+
+
+def test_chain_flops_counts():
+    f = O.chain_flops(65536, 1024, 64)
+    assert abs(f["ssfrk"] - 4.3981e12) / 4.3981e12 < 1e-4
+    assert abs(f["spftrf"] - 9.3827e13) / 9.3827e13 < 1e-4
+    assert abs(f["spftrs"] - 5.4976e11) / 5.4976e11 < 1e-4
