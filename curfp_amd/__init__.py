@@ -49,6 +49,7 @@ try:
         stfttr as _stfttr_C,
         sadd_to_diagonal as _sadd_diag_C,
         sgen_residual as _sgen_residual_C,
+        spftrs_refine as _spftrs_refine_C,
         dsfrk as _dsfrk_C,
         dpftrf as _dpftrf_C,
         dpftrs as _dpftrs_C,
@@ -448,25 +449,23 @@ def dstfttr(arf, *, transr="T", uplo="U", n=None) -> torch.Tensor:
 
 
 def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R=None) -> torch.Tensor:
-    """FP64 residual R = B - (alpha*A@A.T + lam*I) @ X from the FP32 generator A.
+    """FP64 residual R = B - (alpha*op(A)@op(A).T + lam*I) @ X from the FP32 generator A.
 
-    A: float32 (n, k) row-major (trans='T').  X, B: float64 (n, nrhs) — passed
-    to the kernels as column-major nrhs x n... i.e. each is handled through its
-    transpose, which is column-major n x nrhs with leading dimension n when
-    X.t() is contiguous.  For simplicity X and B here are column-major (n,
-    nrhs) tensors: pass ``X.t().contiguous().t()``-style storage, or use
-    :mod:`curfp_amd.refine`, which manages the layouts.
-    """
+    A: float32, row-major (n, k) for trans='T'.  X, B (and R if given):
+    float64 (n, nrhs) tensors stored column-major, i.e. ``t.stride() == (1, n)``
+    (the transpose of a contiguous (nrhs, n) tensor)."""
     _validate(A, "A")
-    n = A.shape[0] if trans.upper() == "T" else A.shape[1]
-    k = A.shape[1] if trans.upper() == "T" else A.shape[0]
-    lda = A.shape[1] if trans.upper() == "T" else A.shape[0]
+    t = trans.upper()
+    n = A.shape[0] if t == "T" else A.shape[1]
+    k = A.shape[1] if t == "T" else A.shape[0]
+    lda = A.shape[1] if t == "T" else A.shape[0]
     nrhs = X.shape[1]
-    for t, nm in ((X, "X"), (B, "B")):
-        if t.dtype != torch.float64 or not t.is_cuda or t.shape != (n, nrhs) or t.stride() != (1, n):
-            raise ValueError(f"{nm} must be a float64 CUDA (n, nrhs) column-major tensor")
     if R is None:
         R = torch.empty_strided((n, nrhs), (1, n), dtype=torch.float64, device=A.device)
+    for v, nm in ((X, "X"), (B, "B"), (R, "R")):
+        if (v.dtype != torch.float64 or not v.is_cuda or tuple(v.shape) != (n, nrhs)
+                or (nrhs > 1 and v.stride() != (1, n)) or (nrhs == 1 and v.stride()[0] != 1)):
+            raise ValueError(f"{nm} must be a float64 CUDA (n, nrhs) column-major tensor")
     _sgen_residual_C(_get_handle(), _op(trans), n, k, nrhs, float(alpha), A.data_ptr(), lda, float(lam),
                      X.data_ptr(), n, B.data_ptr(), n, R.data_ptr(), n)
     return R

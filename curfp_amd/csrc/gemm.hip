@@ -173,7 +173,7 @@ struct SliceLoader {
     }
 };
 
-template <typename T, typename AS, int BM, int BN, int WAVES_M, bool AKC, bool BKC>
+template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M, bool AKC, bool BKC>
 __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
 {
     using MF = Mfma<T>;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
     const int64_t klen = k_hi > k_lo ? k_hi - k_lo : 0;
 
     const AS *A = (const AS *)p.a + k_lo * (AKC ? 1 : p.a_cs);
-    const T *B = (const T *)p.b + k_lo * (BKC ? 1 : p.b_cs);
+    const BS *B = (const BS *)p.b + k_lo * (BKC ? 1 : p.b_cs);
     T *C = (T *)p.c + z * p.c_zstride;
 
     typename MF::acc_t acc[FM][FN];
@@ -216,7 +216,7 @@ __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
 
     const int64_t nk = (klen + BK - 1) / BK;
     SliceLoader<AS, T, BM, BK, AKC> la;
-    SliceLoader<T, T, BN, BK, BKC> lb;
+    SliceLoader<BS, T, BN, BK, BKC> lb;
     if (nk > 0) {
         la.load(A, p.a_off, p.a_rs, p.a_cs, m0, 0, p.M, klen, tid);
         lb.load(B, p.b_off, p.b_rs, p.b_cs, n0, 0, p.N, klen, tid);
@@ -297,7 +297,7 @@ __global__ void splitk_reduce_kernel(const T *P, int64_t M, int64_t N, int64_t s
 /* Host launchers                                                            */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, typename AS, int BM, int BN, int WAVES_M>
+template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M>
 static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
 {
     KParams p;
@@ -340,10 +340,10 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
         p.alpha = 1.0; p.beta = 0.0;
     }
     dim3 grid((unsigned)p.ntiles, (unsigned)splits), block(256);
-    if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BM, BN, WAVES_M, true, true>), grid, block, 0, s, p);
-    else if (akc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BM, BN, WAVES_M, true, false>), grid, block, 0, s, p);
-    else if (bkc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BM, BN, WAVES_M, false, true>), grid, block, 0, s, p);
-    else             hipLaunchKernelGGL((view_gemm_kernel<T, AS, BM, BN, WAVES_M, false, false>), grid, block, 0, s, p);
+    if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, true, true>), grid, block, 0, s, p);
+    else if (akc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, true, false>), grid, block, 0, s, p);
+    else if (bkc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, false, true>), grid, block, 0, s, p);
+    else             hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, false, false>), grid, block, 0, s, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || splits == 1 || p.c != g.partial) return e;
     const int64_t total = g.M * g.N;
@@ -362,18 +362,20 @@ hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s)
     if (math != RFP_MATH_F32 || g.a_bf16) return launch_gemm_bf16(g, math, s);
     /* narrow outputs (the solves' nrhs columns) use a 128 x 64 tile */
     if (!g.lower && g.N <= 64 && g.inplace != 1)
-        return launch_cfg<float, float, 128, 64, 4>(g, s);
-    return launch_cfg<float, float, 128, 128, 2>(g, s);
+        return launch_cfg<float, float, float, 128, 64, 4>(g, s);
+    return launch_cfg<float, float, float, 128, 128, 2>(g, s);
 }
 
 hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     if (g.a_f32) {
+        /* FP32-stored operands, FP64 arithmetic (generator residual / norms) */
         if (g.lower || g.inplace) return hipErrorInvalidValue;
-        return launch_cfg<double, float, 128, 64, 4>(g, s);
+        if (g.b_f32) return launch_cfg<double, float, float, 128, 64, 4>(g, s);
+        return launch_cfg<double, float, double, 128, 64, 4>(g, s);
     }
     if (!g.lower && g.N <= 64 && g.inplace != 1)
-        return launch_cfg<double, double, 128, 64, 4>(g, s);
-    return launch_cfg<double, double, 128, 128, 2>(g, s);
+        return launch_cfg<double, double, double, 128, 64, 4>(g, s);
+    return launch_cfg<double, double, double, 128, 128, 2>(g, s);
 }

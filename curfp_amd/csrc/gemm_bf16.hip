@@ -1,10 +1,299 @@
 /*
- * gemm_bf16.hip — BF16 / split-BF16 MFMA variant of the view GEMM.
+ * gemm_bf16.hip — BF16 and split-BF16 MFMA variants of the view GEMM
+ * (gemm.hip):
+ *
+ *   C(i, j) = alpha * sum_k A(i, k) * B(j, k) + beta * C(i, j)
+ *
+ * used for ssfrk and the spftrf SYRK / GEMM updates when the handle's math
+ * mode is BF16 or BF16X3, and for ssfrk with a BF16 input matrix.
+ *
+ *   BF16   : operands rounded to BF16 (round-to-nearest-even) while they are
+ *            staged into LDS; v_mfma_f32_32x32x16_bf16, FP32 accumulation.
+ *   BF16X3 : x = hi + lo with hi = bf16(x), lo = bf16(x - hi); the product is
+ *            hi*hi + hi*lo + lo*hi (three MFMAs), which keeps ~16 mantissa
+ *            bits of every product — close to FP32 products at BF16 rate / 3.
+ *
+ * Structure: 256 threads (4 wave64s, 2 x 2), 128 x 128 output tile, BK = 32;
+ * every thread stages one 4 x 4 block of each operand per k-slice (loaded
+ * along the view's contiguous direction, transposed in registers) so LDS
+ * holds rows with k contiguous, and each lane feeds the MFMA from one
+ * 16-byte LDS read.  LDS is double-buffered with the next slice's global
+ * loads in flight during the MFMAs.
  */
 #include "rfp_internal.h"
 
+typedef float  f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float  f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned short u16x4u __attribute__((ext_vector_type(4), aligned(2)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+struct BKParams {
+    const void *a; int64_t a_off, a_rs, a_cs;
+    const void *b; int64_t b_off, b_rs, b_cs;
+    float *c;      int64_t c_off, c_rs, c_cs;
+    int64_t M, N, K;
+    float alpha, beta;
+    int64_t tiles_m, tiles_n, ntiles;
+    int lower;
+};
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t pid, int64_t total)
+{
+    const int64_t X = 8;
+    const int64_t q = total / X, r = total % X;
+    const int64_t x = pid % X, w = pid / X;
+    return x * q + (x < r ? x : r) + w;
+}
+
+__device__ __forceinline__ void tile_coords(const BKParams &p, int64_t t, int64_t &tm, int64_t &tn)
+{
+    if (p.lower) {
+        int64_t ti = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+        while (ti * (ti + 1) / 2 > t) --ti;
+        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
+        tm = ti;
+        tn = t - ti * (ti + 1) / 2;
+    } else {
+        const int64_t GROUP = 8;
+        const int64_t in_group = GROUP * p.tiles_n;
+        const int64_t gid = t / in_group;
+        const int64_t first = gid * GROUP;
+        const int64_t gsz = (p.tiles_m - first) < GROUP ? (p.tiles_m - first) : GROUP;
+        const int64_t loc = t % in_group;
+        tm = first + loc % gsz;
+        tn = loc / gsz;
+    }
+}
+
+/* storage element -> float */
+__device__ __forceinline__ float to_f(float x) { return x; }
+__device__ __forceinline__ float to_f(unsigned short x) { return __uint_as_float((unsigned)x << 16); }
+
+__device__ __forceinline__ unsigned short bf16_bits(float x)
+{
+    __bf16 b = (__bf16)x;   /* round to nearest even */
+    return __builtin_bit_cast(unsigned short, b);
+}
+
+template <typename S> struct SVec;
+template <> struct SVec<float> {
+    __device__ static void load4(const float *p, float *o) { f32x4u v = *(const f32x4u *)p; o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; o[3] = v[3]; }
+};
+template <> struct SVec<unsigned short> {
+    __device__ static void load4(const unsigned short *p, float *o)
+    {
+        u16x4u v = *(const u16x4u *)p;
+        o[0] = to_f((unsigned short)v[0]); o[1] = to_f((unsigned short)v[1]);
+        o[2] = to_f((unsigned short)v[2]); o[3] = to_f((unsigned short)v[3]);
+    }
+};
+
+/* One 4 x 4 block (rows r..r+3, k kq..kq+3) per thread: ROWS * BK == 16 * 256. */
+template <typename S, int ROWS, int BK, bool KC>
+struct Stage4x4 {
+    float v[4][4];  /* v[row][k] */
+    int r, kq;
+
+    __device__ __forceinline__ void coords(int tid)
+    {
+        if (KC) { r = (tid / (BK / 4)) * 4; kq = (tid % (BK / 4)) * 4; }
+        else    { r = (tid % (ROWS / 4)) * 4; kq = (tid / (ROWS / 4)) * 4; }
+    }
+
+    __device__ __forceinline__ void load(const S *base, int64_t off, int64_t rs, int64_t cs,
+                                         int64_t r0, int64_t k0, int64_t R, int64_t K)
+    {
+        const int64_t gr = r0 + r, gk = k0 + kq;
+        const bool full = gr + 4 <= R && gk + 4 <= K;
+        if (KC) {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const S *p = base + off + (gr + c) * rs + gk;
+                if (full) {
+                    SVec<S>::load4(p, v[c]);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) v[c][e] = (gr + c < R && gk + e < K) ? to_f(p[e]) : 0.f;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const S *p = base + off + gr + (gk + e) * cs;
+                float t[4];
+                if (full) {
+                    SVec<S>::load4(p, t);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) t[c] = (gr + c < R && gk + e < K) ? to_f(p[c]) : 0.f;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) v[c][e] = t[c];
+            }
+        }
+    }
+
+    /* rows with k contiguous, pitch P bf16 elements */
+    template <int P, bool X3>
+    __device__ __forceinline__ void store(unsigned short *hi, unsigned short *lo) const
+    {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            u16x4 h, l;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const unsigned short hb = bf16_bits(v[c][e]);
+                h[e] = hb;
+                if (X3) l[e] = bf16_bits(v[c][e] - to_f(hb));
+            }
+            *(u16x4 *)(hi + (r + c) * P + kq) = h;
+            if (X3) *(u16x4 *)(lo + (r + c) * P + kq) = l;
+        }
+    }
+};
+
+template <typename S, bool X3, bool AKC, bool BKC>
+__global__ __launch_bounds__(256) void view_gemm_bf16_kernel(BKParams p)
+{
+    constexpr int BM = 128, BN = 128, BK = 32, P = BK + 8;
+    constexpr int NB = X3 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][NB][BM * P];
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][NB][BN * P];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    int64_t tm, tn;
+    tile_coords(p, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
+    const int64_t m0 = tm * BM, n0 = tn * BN;
+    const S *A = (const S *)p.a;
+    const S *B = (const S *)p.b;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    Stage4x4<S, BM, BK, AKC> la;
+    Stage4x4<S, BN, BK, BKC> lb;
+    la.coords(tid);
+    lb.coords(tid);
+    const int64_t nk = (p.K + BK - 1) / BK;
+    if (nk > 0) {
+        la.load(A, p.a_off, p.a_rs, p.a_cs, m0, 0, p.M, p.K);
+        lb.load(B, p.b_off, p.b_rs, p.b_cs, n0, 0, p.N, p.K);
+        la.template store<P, X3>(sA[0][0], sA[0][NB - 1]);
+        lb.template store<P, X3>(sB[0][0], sB[0][NB - 1]);
+    }
+    __syncthreads();
+
+    const int lrow = lane & 31, lk = (lane >> 5) * 8;
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            la.load(A, p.a_off, p.a_rs, p.a_cs, m0, (kt + 1) * BK, p.M, p.K);
+            lb.load(B, p.b_off, p.b_rs, p.b_cs, n0, (kt + 1) * BK, p.N, p.K);
+        }
+#pragma unroll
+        for (int k = 0; k < BK; k += 16) {
+            bf16x8 ah[2], bh[2], al[2], bl[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+                const int off = (wm * 64 + i * 32 + lrow) * P + k + lk;
+                ah[i] = *(const bf16x8 *)(sA[cur][0] + off);
+                if (X3) al[i] = *(const bf16x8 *)(sA[cur][NB - 1] + off);
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int off = (wn * 64 + j * 32 + lrow) * P + k + lk;
+                bh[j] = *(const bf16x8 *)(sB[cur][0] + off);
+                if (X3) bl[j] = *(const bf16x8 *)(sB[cur][NB - 1] + off);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j) {
+                    if (X3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al[i], bh[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bl[j], acc[i][j], 0, 0, 0);
+                    }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah[i], bh[j], acc[i][j], 0, 0, 0);
+                }
+        }
+        if (more) {
+            la.template store<P, X3>(sA[cur ^ 1][0], sA[cur ^ 1][NB - 1]);
+            lb.template store<P, X3>(sB[cur ^ 1][0], sB[cur ^ 1][NB - 1]);
+        }
+        __syncthreads();
+    }
+
+    const bool use_beta = p.beta != 0.f;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t gj = n0 + wn * 64 + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
+                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
+                    float val = p.alpha * acc[i][j][r];
+                    if (use_beta) val += p.beta * (*cp);
+                    *cp = val;
+                }
+            }
+        }
+}
+
+template <typename S, bool X3>
+hipError_t launch_typed(const BKParams &p, bool akc, bool bkc, hipStream_t s)
+{
+    dim3 grid((unsigned)p.ntiles), block(256);
+    if (akc && bkc)  hipLaunchKernelGGL((view_gemm_bf16_kernel<S, X3, true, true>), grid, block, 0, s, p);
+    else if (akc)    hipLaunchKernelGGL((view_gemm_bf16_kernel<S, X3, true, false>), grid, block, 0, s, p);
+    else if (bkc)    hipLaunchKernelGGL((view_gemm_bf16_kernel<S, X3, false, true>), grid, block, 0, s, p);
+    else             hipLaunchKernelGGL((view_gemm_bf16_kernel<S, X3, false, false>), grid, block, 0, s, p);
+    return hipGetLastError();
+}
+
+}  // namespace
+
 hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
 {
-    (void)g; (void)math; (void)s;
-    return hipErrorNotSupported;
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    if (g.inplace || g.splits > 1) return hipErrorInvalidValue;
+    BKParams p;
+    p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;
+    p.b = g.b; p.b_off = g.vb.off; p.b_rs = g.vb.rs; p.b_cs = g.vb.cs;
+    p.c = (float *)g.c; p.c_off = g.vc.off; p.c_rs = g.vc.rs; p.c_cs = g.vc.cs;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.alpha = (float)g.alpha; p.beta = (float)g.beta;
+    p.lower = g.lower ? 1 : 0;
+    p.tiles_m = (g.M + 127) / 128;
+    p.tiles_n = (g.N + 127) / 128;
+    if (g.lower) {
+        if (g.M != g.N) return hipErrorInvalidValue;
+        p.ntiles = p.tiles_m * (p.tiles_m + 1) / 2;
+    } else {
+        p.ntiles = p.tiles_m * p.tiles_n;
+    }
+    if (p.ntiles <= 0) return hipSuccess;
+    if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    const bool akc = g.va.cs == 1, bkc = g.vb.cs == 1;
+    if (g.K > 0 && ((!akc && g.va.rs != 1) || (!bkc && g.vb.rs != 1))) return hipErrorInvalidValue;
+    /* a BF16 input is exact in BF16: split-BF16 adds nothing there */
+    const bool x3 = math == RFP_MATH_BF16X3 && !g.a_bf16;
+    if (g.a_bf16) return launch_typed<unsigned short, false>(p, akc, bkc, s);
+    if (x3) return launch_typed<float, true>(p, akc, bkc, s);
+    return launch_typed<float, false>(p, akc, bkc, s);
 }

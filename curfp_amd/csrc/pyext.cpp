@@ -135,6 +135,17 @@ PYBIND11_MODULE(_curfp_C, m)
                                        P<const double>(X), ldx, P<const double>(B), ldb, P<double>(R), ldr));
     });
 
+    m.def("spftrs_refine", [](Handle &h, int transr, int uplo, int trans, int n, int k, int nrhs, double alpha,
+                              int64_t A, int lda, double lambda, int64_t F, int64_t B, int ldb, int64_t X, int ldx,
+                              double tol, int max_iter) {
+        int iters = 0;
+        double resid = 0.0;
+        check_status(curfpSpftrsRefine(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, nrhs, &alpha,
+                                       P<const float>(A), lda, &lambda, P<const float>(F), P<const double>(B), ldb,
+                                       P<double>(X), ldx, tol, max_iter, &iters, &resid));
+        return py::make_tuple(iters, resid);
+    });
+
     /* ---- FP64 ---- */
     m.def("dsfrk", [](Handle &h, int transr, int uplo, int trans, int n, int k, double alpha, int64_t A, int lda,
                       double beta, int64_t C) {

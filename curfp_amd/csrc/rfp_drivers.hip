@@ -25,7 +25,9 @@
  * of being allocated per call (src/curfp_spftrf.cpp:231-232).
  */
 #include "rfp_internal.h"
+#include <algorithm>
 #include <climits>
+#include <cmath>
 #include <new>
 
 /* ------------------------------------------------------------------------ */
@@ -467,6 +469,36 @@ struct Solver {
     }
 };
 
+static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
+{
+    const int64_t leaf = rfp_leaf(f64);
+    const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
+    return (size_t)(s1 + s2) * leaf * leaf * (f64 ? sizeof(double) : sizeof(float));
+}
+
+/* Forward + backward substitution on B (view bv, n x nrhs) with the RFP
+ * factor A; `inv` holds pftrs_scratch_bytes of scratch for the leaf inverses. */
+template <typename T>
+static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B, RfpView bv, int64_t nrhs, T *inv)
+{
+    const int64_t leaf = rfp_leaf(Prec<T>::f64);
+    const int64_t s1 = (g.n1 + leaf - 1) / leaf;
+    RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
+                  [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
+    RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
+                  [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
+    Solver<T> sv{h, A, B, inv, leaf, nrhs};
+    const RfpView b2 = bv.sub(g.n1, 0);
+    /* forward: L y = b */
+    RFP_TRY(sv.lower(g.v11, g.n1, bv, 0));
+    if (g.n1 > 0 && g.n2 > 0) RFP_TRY(sv.gemm(b2, A, g.v21, B, bv.t(), g.n2, g.n1, -1.0, 1.0, 0));
+    RFP_TRY(sv.lower(g.v22, g.n2, b2, s1));
+    /* backward: L^T x = y */
+    RFP_TRY(sv.lower_t(g.v22, g.n2, b2, s1));
+    if (g.n1 > 0 && g.n2 > 0) RFP_TRY(sv.gemm(bv, A, g.v21.t(), B, b2.t(), g.n1, g.n2, -1.0, 1.0, 0));
+    return sv.lower_t(g.v11, g.n1, bv, 0);
+}
+
 template <typename T>
 static curfpStatus_t pftrs_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
                                 int nrhs, const T *A, T *B, int ldb, bool row_major)
@@ -479,32 +511,12 @@ static curfpStatus_t pftrs_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
     if (!A || !B || ldb < (row_major ? nrhs : n)) return CURFP_STATUS_INVALID_VALUE;
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
-    const int64_t leaf = rfp_leaf(Prec<T>::f64);
-    const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
     void *ws = nullptr;
-    hipError_t e = rfp_workspace(h, (size_t)(s1 + s2) * leaf * leaf * sizeof(T), &ws);
+    hipError_t e = rfp_workspace(h, pftrs_scratch_bytes(g, Prec<T>::f64), &ws);
     if (e != hipSuccess) return st_of(e);
-    T *inv = (T *)ws;
-    e = timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
-              [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); });
-    if (e == hipSuccess)
-        e = timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
-                  [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); });
-    if (e != hipSuccess) return st_of(e);
-
-    Solver<T> sv{h, A, B, inv, leaf, nrhs};
     /* B(i, c): column-major n x nrhs (the reference ABI) or row-major */
     const RfpView bv = row_major ? RfpView{0, ldb, 1} : RfpView{0, 1, ldb};
-    const RfpView b2 = bv.sub(g.n1, 0);
-    /* forward: L y = b */
-    e = sv.lower(g.v11, g.n1, bv, 0);
-    if (e == hipSuccess && g.n1 > 0 && g.n2 > 0) e = sv.gemm(b2, A, g.v21, B, bv.t(), g.n2, g.n1, -1.0, 1.0, 0);
-    if (e == hipSuccess) e = sv.lower(g.v22, g.n2, b2, s1);
-    /* backward: L^T x = y */
-    if (e == hipSuccess) e = sv.lower_t(g.v22, g.n2, b2, s1);
-    if (e == hipSuccess && g.n1 > 0 && g.n2 > 0) e = sv.gemm(bv, A, g.v21.t(), B, b2.t(), g.n1, g.n2, -1.0, 1.0, 0);
-    if (e == hipSuccess) e = sv.lower_t(g.v11, g.n1, bv, 0);
-    return st_of(e);
+    return st_of(pftrs_core<T>(h, g, A, B, bv, nrhs, (T *)ws));
 }
 
 /* ------------------------------------------------------------------------ */
@@ -574,6 +586,51 @@ __global__ void residual_init_kernel(int64_t n, int64_t nrhs, double lambda, con
     R[i + c * ldr] = B[i + c * ldb] - lambda * X[i + c * ldx];
 }
 
+static int64_t residual_splits(int64_t n) { return n >= 8192 ? 32 : 1; }
+
+static size_t residual_scratch_bytes(int64_t n, int64_t k, int64_t nrhs)
+{
+    return (size_t)k * nrhs * (1 + residual_splits(n)) * sizeof(double);
+}
+
+/* R = B - (alpha op(A) op(A)^T + lambda I) X in FP64; `tmp` has
+ * residual_scratch_bytes of scratch. */
+static hipError_t gen_residual_core(curfpHandle_t h, bool trans_n, int64_t n, int64_t k, int64_t nrhs, double alpha,
+                                    const float *A, int64_t lda, double lambda, const double *X, int64_t ldx,
+                                    const double *B, int64_t ldb, double *R, int64_t ldr, double *tmp)
+{
+    const int64_t total = n * nrhs;
+    RFP_TRY(timed(h, CURFP_KERNEL_OTHER, 0.0, 24.0 * total, [&] {
+        hipLaunchKernelGGL(residual_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream, n,
+                           nrhs, lambda, X, ldx, B, ldb, R, ldr);
+        return hipGetLastError();
+    }));
+    if (k == 0 || alpha == 0.0) return hipSuccess;
+    /* T = op(A)^T X (k x nrhs), split-K over n; then R -= alpha op(A) T */
+    const int64_t splits = residual_splits(n);
+    const size_t t_elems = (size_t)k * nrhs;
+    double *T = tmp;
+    const RfpView av = trans_n ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
+    GemmArgs g1 = gemm_args();
+    g1.a = A; g1.va = av.t();
+    g1.b = X; g1.vb = RfpView{0, ldx, 1};
+    g1.c = T; g1.vc = RfpView{0, 1, k};
+    g1.M = k; g1.N = nrhs; g1.K = n;
+    g1.alpha = 1.0; g1.beta = 0.0;
+    g1.a_f32 = true;
+    g1.splits = (int)splits;
+    g1.partial = T + t_elems;
+    RFP_TRY(run_gemm<double>(h, g1, RFP_MATH_F64));
+    GemmArgs g2 = gemm_args();
+    g2.a = A; g2.va = av;
+    g2.b = T; g2.vb = RfpView{0, k, 1};
+    g2.c = R; g2.vc = RfpView{0, 1, ldr};
+    g2.M = n; g2.N = nrhs; g2.K = k;
+    g2.alpha = -alpha; g2.beta = 1.0;
+    g2.a_f32 = true;
+    return run_gemm<double>(h, g2, RFP_MATH_F64);
+}
+
 extern "C" curfpStatus_t curfpSgenResidual(curfpHandle_t h, curfpOperation_t trans, int n, int k, int nrhs,
                                            const double *alpha, const float *A, int lda, const double *lambda,
                                            const double *X, int ldx, const double *B, int ldb, double *R,
@@ -585,42 +642,112 @@ extern "C" curfpStatus_t curfpSgenResidual(curfpHandle_t h, curfpOperation_t tra
     if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
     if (!X || !B || !R || ldx < n || ldb < n || ldr < n) return CURFP_STATUS_INVALID_VALUE;
     if (k > 0 && !A) return CURFP_STATUS_INVALID_VALUE;
-    /* R = B - lambda X */
-    {
-        const int64_t total = (int64_t)n * nrhs;
-        hipLaunchKernelGGL(residual_init_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, h->stream,
-                           (int64_t)n, (int64_t)nrhs, *lambda, X, (int64_t)ldx, B, (int64_t)ldb, R, (int64_t)ldr);
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return st_of(e);
-    }
-    if (k == 0 || *alpha == 0.0) return CURFP_STATUS_SUCCESS;
-    /* T = op(A)^T X (k x nrhs), split-K over n; then R -= alpha op(A) T */
-    const int64_t splits = n >= 8192 ? 32 : 1;
-    const size_t t_elems = (size_t)k * nrhs;
     void *ws = nullptr;
-    hipError_t e = rfp_workspace(h, (t_elems * (1 + splits)) * sizeof(double), &ws);
+    hipError_t e = rfp_workspace(h, residual_scratch_bytes(n, k, nrhs), &ws);
     if (e != hipSuccess) return st_of(e);
-    double *T = (double *)ws;
-    const RfpView av = (trans == CURFP_OP_N) ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
-    GemmArgs g1 = gemm_args();
-    g1.a = A; g1.va = av.t();
-    g1.b = X; g1.vb = RfpView{0, ldx, 1};
-    g1.c = T; g1.vc = RfpView{0, 1, k};
-    g1.M = k; g1.N = nrhs; g1.K = n;
-    g1.alpha = 1.0; g1.beta = 0.0;
-    g1.a_f32 = true;
-    g1.splits = (int)splits;
-    g1.partial = T + t_elems;
-    e = run_gemm<double>(h, g1, RFP_MATH_F64);
+    return st_of(gen_residual_core(h, trans == CURFP_OP_N, n, k, nrhs, *alpha, A, lda, *lambda, X, ldx, B, ldb, R,
+                                   ldr, (double *)ws));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mixed-precision iterative refinement                                      */
+/* ------------------------------------------------------------------------ */
+
+static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo,
+                                           curfpOperation_t trans, int n, int k, int nrhs, const double *alpha,
+                                           const float *A, int lda, const double *lambda, const float *F,
+                                           const double *B, int ldb, double *X, int ldx, double tol, int max_iter,
+                                           int *iters, double *resid)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || k < 0 || nrhs < 0 || lda < 1 || !alpha || !lambda || max_iter < 1 || !(tol >= 0.0))
+        return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) || (trans != CURFP_OP_N && trans != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (iters) *iters = 0;
+    if (resid) *resid = 0.0;
+    if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
+    if (!F || !B || !X || ldb < n || ldx < n || (k > 0 && !A)) return CURFP_STATUS_INVALID_VALUE;
+
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const int64_t N = n, NR = nrhs, K = k;
+    const int64_t gsplit = N >= 8192 ? 8 : 1;
+    /* scratch shared by the solves / residual / Gram product, then the
+     * refinement's own buffers */
+    size_t shared = pftrs_scratch_bytes(g, false);
+    shared = std::max(shared, residual_scratch_bytes(N, K, NR));
+    shared = std::max(shared, (size_t)K * K * (1 + gsplit) * sizeof(double));
+    shared = align256(shared);
+    const size_t r_bytes = align256((size_t)N * NR * sizeof(double));
+    const size_t d_bytes = align256((size_t)N * NR * sizeof(float));
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, shared + r_bytes + d_bytes + 256, &ws);
     if (e != hipSuccess) return st_of(e);
-    GemmArgs g2 = gemm_args();
-    g2.a = A; g2.va = av;
-    g2.b = T; g2.vb = RfpView{0, k, 1};
-    g2.c = R; g2.vc = RfpView{0, 1, ldr};
-    g2.M = n; g2.N = nrhs; g2.K = k;
-    g2.alpha = -*alpha; g2.beta = 1.0;
-    g2.a_f32 = true;
-    return st_of(run_gemm<double>(h, g2, RFP_MATH_F64));
+    char *base = (char *)ws;
+    double *R = (double *)(base + shared);
+    float *D = (float *)(base + shared + r_bytes);
+    double *sums = (double *)(base + shared + r_bytes + d_bytes);  /* [0]=||R||^2 [1]=||X||^2 [2..3]=Gram */
+    const double a = *alpha, lam = *lambda;
+    const RfpView dv{0, 1, N};
+    hipStream_t s = h->stream;
+
+    /* ||C||_F^2 = a^2 ||A^T A||_F^2 + 2 a lam ||A||_F^2 + n lam^2, via G = op(A)^T op(A) */
+    double normC2 = (double)N * lam * lam;
+    if (K > 0 && a != 0.0) {
+        double *G = (double *)base;
+        const RfpView av = trans == CURFP_OP_N ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
+        GemmArgs gg = gemm_args();
+        gg.a = A; gg.va = av.t();
+        gg.b = A; gg.vb = av.t();
+        gg.c = G; gg.vc = RfpView{0, 1, K};
+        gg.M = K; gg.N = K; gg.K = N;
+        gg.alpha = 1.0; gg.beta = 0.0;
+        gg.a_f32 = true; gg.b_f32 = true;
+        gg.splits = (int)gsplit;
+        gg.partial = G + K * K;
+        if ((e = launch_zero_f64(sums, 4, s)) != hipSuccess) return st_of(e);
+        if ((e = run_gemm<double>(h, gg, RFP_MATH_F64)) != hipSuccess) return st_of(e);
+        if ((e = launch_gram_norms(K, G, sums + 2, s)) != hipSuccess) return st_of(e);
+        double hs[2];
+        if ((e = hipMemcpyAsync(hs, sums + 2, sizeof(hs), hipMemcpyDeviceToHost, s)) != hipSuccess) return st_of(e);
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return st_of(e);
+        normC2 += a * a * hs[0] + 2.0 * a * lam * hs[1];
+    }
+    const double normC = sqrt(normC2);
+
+    /* X0 = F^{-1} B in FP32 */
+    int it = 0;
+    double rel = INFINITY;
+    if ((e = launch_cast_f64_f32(N, NR, B, ldb, D, N, s)) != hipSuccess) return st_of(e);
+    if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
+    if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, false, s)) != hipSuccess) return st_of(e);
+    it = 1;
+    for (;;) {
+        /* R = B - C X (FP64), relative residual */
+        e = gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, X, ldx, B, ldb, R, N,
+                              (double *)base);
+        if (e == hipSuccess) e = launch_zero_f64(sums, 2, s);
+        if (e == hipSuccess) e = launch_sumsq(N, NR, R, N, sums, s);
+        if (e == hipSuccess) e = launch_sumsq(N, NR, X, ldx, sums + 1, s);
+        double hs[2] = {0.0, 0.0};
+        if (e == hipSuccess) e = hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return st_of(e);
+        const double nx = sqrt(hs[1]);
+        rel = (normC > 0.0 && nx > 0.0) ? sqrt(hs[0]) / (normC * nx) : sqrt(hs[0]);
+        if (!(rel > tol) || it >= max_iter || !std::isfinite(rel)) break;
+        /* D = F^{-1} fp32(R); X += D */
+        if ((e = launch_cast_f64_f32(N, NR, R, N, D, N, s)) != hipSuccess) return st_of(e);
+        if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
+        if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, true, s)) != hipSuccess) return st_of(e);
+        ++it;
+    }
+    if (iters) *iters = it;
+    if (resid) *resid = rel;
+    return CURFP_STATUS_SUCCESS;
 }
 
 /* ------------------------------------------------------------------------ */

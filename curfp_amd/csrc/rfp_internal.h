@@ -151,6 +151,7 @@ struct GemmArgs {
     int  inplace;      /* 0 none, 1: C aliases A (one WG spans all N), 2: C aliases B (one WG spans all M) */
     bool a_bf16;       /* A and B are raw BF16 storage (ssfrk with BF16 input) */
     bool a_f32;        /* FP64 GEMM whose A is stored as FP32 (generator residual) */
+    bool b_f32;        /* ... and whose B is stored as FP32 too (requires a_f32) */
     int  splits;       /* split-K slices (needs `partial`: splits*M*N elements) */
     void *partial;
 };
@@ -196,8 +197,11 @@ hipError_t launch_add_diag_f64(const RfpGeom &g, double value, double *arf, hipS
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 hipError_t launch_set_int(int *p, int value, hipStream_t s);
 
-/* FP64 generator residual, see curfpSgenResidual. */
-hipError_t launch_gen_residual(bool trans_n, int64_t n, int64_t k, int64_t nrhs, double alpha,
-                               const float *A, int64_t lda, double lambda,
-                               const double *X, int64_t ldx, const double *B, int64_t ldb,
-                               double *R, int64_t ldr, double *tmp, hipStream_t s);
+/* Iterative-refinement helpers (refine.hip). */
+hipError_t launch_cast_f64_f32(int64_t n, int64_t nrhs, const double *src, int64_t lds, float *dst, int64_t ldd,
+                               hipStream_t s);
+hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t ldd, double *X, int64_t ldx,
+                              bool accumulate, hipStream_t s);
+hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s);
+hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_t s);
+hipError_t launch_zero_f64(double *p, int n, hipStream_t s);

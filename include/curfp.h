@@ -184,6 +184,27 @@ curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelClass_t cls,
                                   int64_t *launches, double *flops,
                                   double *bytes, double *milliseconds, int reset);
 
+/*
+ * curfpSpftrsRefine — mixed-precision iterative refinement (no reference
+ * counterpart) for
+ *     C X = B,   C = alpha * op(A) * op(A)^T + lambda * I   (n x n),
+ * given the FP32 RFP Cholesky factor F of C (from curfpSpftrf, in any math
+ * mode) and the FP32 generator A (trans / lda as in curfpSsfrk).
+ *   X_0 = F^{-1} B;  repeat  R = B - C X (FP64, from the generator),
+ *   X += F^{-1} fp32(R)  (FP64 accumulation)
+ * until ||R||_F <= tol * ||C||_F * ||X||_F or max_iter solves.
+ * B and X are FP64 column-major n x nrhs (ldb, ldx >= n); X need not be
+ * initialised.  *iters = solves done, *resid = final relative residual.
+ * Host-synchronous (reads the residual norm once per iteration).
+ */
+curfpStatus_t curfpSpftrsRefine(curfpHandle_t handle, curfpOperation_t transr,
+                                curfpFillMode_t uplo, curfpOperation_t trans,
+                                int n, int k, int nrhs, const double *alpha,
+                                const float *A, int lda, const double *lambda,
+                                const float *F, const double *B, int ldb,
+                                double *X, int ldx, double tol, int max_iter,
+                                int *iters, double *resid);
+
 /* ---------------------------- FP64 twins -------------------------------- */
 
 /* curfpDsfrk  — reference include/curfp.h:357-369, src/curfp_dssfrk.cpp:101-172. */

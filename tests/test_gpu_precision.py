@@ -1,0 +1,151 @@
+"""GPU tests of the precision modes and the FP64 iterative refinement:
+BF16 / split-BF16 ssfrk and spftrf updates, the FP64 generator residual, and
+refinement down to a relative residual of 1e-6 (the north-star target) and
+below, checked against float64 dense arithmetic on the host."""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import rfp_oracle as O
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [(t, u) for t in "NT" for u in "LU"]
+
+
+@pytest.fixture(scope="module")
+def curfp(cuda):
+    import curfp_amd
+
+    return curfp_amd
+
+
+def bf16_round(x: np.ndarray) -> np.ndarray:
+    """Round-to-nearest-even to BF16, returned as float64 (via torch)."""
+    return torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).to(torch.bfloat16).double().numpy()
+
+
+def problem(n, k, nrhs, seed, lam=1.0):
+    rng = np.random.default_rng(seed)
+    A = (rng.standard_normal((n, k)) / math.sqrt(k)).astype(np.float32)
+    B = rng.standard_normal((n, nrhs)).astype(np.float32)
+    return A, B
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("n,k", [(600, 130), (1001, 257)])
+def test_ssfrk_bf16_input_is_exact_products(curfp, t, u, n, k):
+    """A in BF16: every product is exact in FP32, so C matches the float64
+    product of the BF16 values to FP32 accumulation accuracy."""
+    A, _ = problem(n, k, 1, seed=n + k)
+    Ab = torch.from_numpy(A).cuda().to(torch.bfloat16)
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(Ab, C, transr=t, uplo=u)
+    A64 = Ab.double().cpu().numpy()
+    S = A64 @ A64.T
+    Cd = O.rfp_to_dense_sym(C.cpu().numpy().astype(np.float64), n, t, u)
+    u32 = np.finfo(np.float32).eps
+    assert np.linalg.norm(Cd - S) <= 2 * math.sqrt(k) * u32 * np.linalg.norm(S) + 1e-30
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_ssfrk_math_modes(curfp, t, u):
+    n, k = 777, 300
+    A, _ = problem(n, k, 1, seed=11)
+    Ad = torch.from_numpy(A).cuda()
+    S = A.astype(np.float64) @ A.astype(np.float64).T
+    Sb = bf16_round(A) @ bf16_round(A).T
+    u32 = np.finfo(np.float32).eps
+    res = {}
+    for mode in ("fp32", "bf16", "bf16x3"):
+        C = torch.empty(n * (n + 1) // 2, device="cuda")
+        with curfp.math_mode(mode):
+            curfp.ssfrk(Ad, C, transr=t, uplo=u)
+        res[mode] = O.rfp_to_dense_sym(C.cpu().numpy().astype(np.float64), n, t, u)
+    nS = np.linalg.norm(S)
+    # BF16 mode = exact product of the BF16-rounded inputs
+    assert np.linalg.norm(res["bf16"] - Sb) <= 4 * math.sqrt(k) * u32 * np.linalg.norm(Sb)
+    # split-BF16 recovers ~FP32 accuracy; FP32 is FP32
+    assert np.linalg.norm(res["bf16x3"] - S) <= 1e-5 * nS
+    assert np.linalg.norm(res["fp32"] - S) <= 4 * math.sqrt(k) * u32 * nS
+    assert np.linalg.norm(res["bf16"] - S) > np.linalg.norm(res["bf16x3"] - S)
+
+
+@pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
+@pytest.mark.parametrize("n,t,u", [(2048, "T", "U"), (2047, "N", "L"), (1500, "N", "U"), (1501, "T", "L")])
+def test_spftrf_math_modes_backward_error(curfp, mode, n, t, u):
+    """Factor with BF16 / split-BF16 updates: ||L L^T - C|| / ||C|| at the
+    precision of the mode (BF16 ~ 2^-8 scale, split-BF16 ~ FP32 scale)."""
+    k = 256
+    A, _ = problem(n, k, 1, seed=3)
+    Ad = torch.from_numpy(A).cuda()
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(Ad, C, transr=t, uplo=u)
+    curfp.add_to_diagonal(C, 1.0, transr=t, uplo=u)
+    S = O.rfp_to_dense_sym(C.cpu().numpy().astype(np.float64), n, t, u)
+    with curfp.math_mode(mode):
+        assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    L = O.rfp_lower_factor(C.cpu().numpy().astype(np.float64), n, t, u)
+    err = np.linalg.norm(L @ L.T - S) / np.linalg.norm(S)
+    bound = 2.0 ** -7 if mode == "bf16" else 1e-5
+    assert err <= bound, err
+
+
+def test_gen_residual_fp64(curfp):
+    n, k, nrhs = 9000, 150, 7
+    A, B = problem(n, k, nrhs, seed=5)
+    rng = np.random.default_rng(6)
+    X = rng.standard_normal((n, nrhs))
+    lam = 0.3
+    Ad = torch.from_numpy(A).cuda()
+    Xc = torch.from_numpy(X.T.copy()).cuda().t()
+    Bc = torch.from_numpy(B.astype(np.float64).T.copy()).cuda().t()
+    R = curfp.gen_residual(Ad, Xc, Bc, lam=lam).cpu().numpy()
+    A64 = A.astype(np.float64)
+    Rref = B.astype(np.float64) - (A64 @ (A64.T @ X) + lam * X)
+    assert np.abs(R - Rref).max() <= 1e-12 * np.abs(Rref).max() + 1e-12
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_refinement_reaches_target(curfp, mode, t, u):
+    """BF16 factor + FP64 refinement reaches ||B - C X|| / (||C|| ||X||) <= 1e-6,
+    verified independently in float64 on the host."""
+    n, k, nrhs, lam = 2047 if t == "N" else 2048, 256, 16, 1.0
+    A, B = problem(n, k, nrhs, seed=21)
+    Ad = torch.from_numpy(A).cuda()
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    with curfp.math_mode(mode):
+        curfp.ssfrk(Ad, C, transr=t, uplo=u)
+        curfp.add_to_diagonal(C, lam, transr=t, uplo=u)
+        assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    from curfp_amd import refine
+
+    X, info = refine.refine_solve(C, Ad, lam, torch.from_numpy(B).cuda(), tol=1e-6, transr=t, uplo=u)
+    S = A.astype(np.float64) @ A.astype(np.float64).T + lam * np.eye(n)
+    r = O.dense_relative_residual(S, X.cpu().numpy(), B.astype(np.float64))
+    assert info["rel_residual"] <= 1e-6 and r <= 1.5e-6, (info, r)
+    assert abs(r - info["rel_residual"]) <= 0.05 * r + 1e-16
+    if mode == "fp32":
+        assert info["solves"] <= 2
+
+
+def test_refinement_to_fp64_accuracy(curfp):
+    """Driving the tolerance down shows the FP64 accumulation: the error
+    against a float64 dense solve falls far below FP32 roundoff."""
+    n, k, nrhs, lam = 1500, 200, 4, 0.5
+    A, B = problem(n, k, nrhs, seed=8)
+    Ad = torch.from_numpy(A).cuda()
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(Ad, C)
+    curfp.add_to_diagonal(C, lam)
+    curfp.spftrf(C)
+    from curfp_amd import refine
+
+    X, info = refine.refine_solve(C, Ad, lam, torch.from_numpy(B).cuda(), tol=1e-14, max_iter=20)
+    S = A.astype(np.float64) @ A.astype(np.float64).T + lam * np.eye(n)
+    Xref = np.linalg.solve(S, B.astype(np.float64))
+    err = np.linalg.norm(X.cpu().numpy() - Xref) / np.linalg.norm(Xref)
+    assert info["rel_residual"] <= 1e-14 and err <= 1e-12, (info, err)
