@@ -57,7 +57,9 @@ template <> struct Mfma<double> {
     __device__ static acc_t mma(double a, double b, acc_t c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
-    __device__ static int row(int l, int r) { return 4 * (l >> 4) + r; }
+    /* FP64 output: register r of lane l holds row (l / 16) + 4 r — unlike the
+     * FP32 16x16 shapes, whose rows are 4 (l / 16) + r (checked on gfx950) */
+    __device__ static int row(int l, int r) { return (l >> 4) + 4 * r; }
     __device__ static int col(int l) { return l & 15; }
     __device__ static int kk(int l) { return l >> 4; }
     __device__ static int mn(int l) { return l & 15; }

@@ -63,45 +63,158 @@ __device__ __forceinline__ void store_lower(const T *s, T *base, RfpView v, int 
 }
 
 /*
- * In-place inverse of the m x m lower-triangular matrix in s (pitch P),
- * by rows: X(i, c) = -(sum_{c <= k < i} L(i, k) X(k, c)) / L(i, i).
- * Row i of L is consumed while row i of X is produced, so the new row goes
- * through `row` first.  Two threads share each column c (halves of the k
- * range) and combine through `part`.
+ * Register-blocked tile: a 16 x 16 grid of threads, thread (ty, tx) owning the
+ * RB x RB block of rows ty*RB.., columns tx*RB.. (RB = NB / 16).  All
+ * register arrays are indexed with compile-time indices only; runtime
+ * positions (the current column j) are matched with selects.
  */
-template <typename T, int NB, int P>
-__device__ void invert_lower(T *s, T *row, T *part, int m, int tid)
-{
-    const int c = tid % NB;
-    const int half = tid / NB;           /* 0 or 1 when NB == 128; NB == 64 uses 4 quarters */
-    constexpr int SPLIT = 256 / NB;
-    for (int i = 0; i < m; ++i) {
-        T sum = 0;
-        if (c < i) {
-            const int len = i - c;
-            const int chunk = (len + SPLIT - 1) / SPLIT;
-            const int k0 = c + half * chunk;
-            const int k1 = min(i, k0 + chunk);
-            for (int k = k0; k < k1; ++k) sum += s[i * P + k] * s[k * P + c];
-        }
-        part[half * NB + c] = sum;
-        __syncthreads();
-        if (half == 0 && c <= i) {
-            const T lii = s[i * P + i];
-            T v;
-            if (c == i) {
-                v = T(1) / lii;
-            } else {
-                T tot = 0;
+template <typename T, int NB>
+struct RegTile {
+    static constexpr int RB = NB / 16;
+    int r0, c0, tx, ty;
+
+    __device__ void init(int tid)
+    {
+        tx = tid & 15;
+        ty = tid >> 4;
+        r0 = ty * RB;
+        c0 = tx * RB;
+    }
+
+    template <int P>
+    __device__ void from_lds(const T *s, T (&a)[RB][RB]) const
+    {
 #pragma unroll
-                for (int h = 0; h < SPLIT; ++h) tot += part[h * NB + c];
-                v = -tot / lii;
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < RB; ++c) a[r][c] = s[(r0 + r) * P + c0 + c];
+    }
+
+    template <int P>
+    __device__ void to_lds(T *s, const T (&a)[RB][RB]) const
+    {
+#pragma unroll
+        for (int r = 0; r < RB; ++r)
+#pragma unroll
+            for (int c = 0; c < RB; ++c) s[(r0 + r) * P + c0 + c] = a[r][c];
+    }
+
+    /* column jj (runtime, 0 <= jj < RB) of this thread's block */
+    __device__ T pick_col(const T (&a)[RB][RB], int r, int jj) const
+    {
+        T v = 0;
+#pragma unroll
+        for (int c = 0; c < RB; ++c) v = (c == jj) ? a[r][c] : v;
+        return v;
+    }
+};
+
+/*
+ * Right-looking Cholesky of the m x m lower tile held in registers.  Step j:
+ * the threads owning column j publish it to LDS (double-buffered, so one
+ * barrier per step), every thread reads the pivot and the 2*RB column
+ * entries it needs, scales and applies the rank-1 update to its block.
+ * Returns the failing column (pivot not > 0, or NaN), or -1.
+ */
+template <typename T, int NB>
+__device__ int chol_regs(const RegTile<T, NB> &rt, T (&a)[NB / 16][NB / 16], T (*colbuf)[NB], int m)
+{
+    constexpr int RB = NB / 16;
+    for (int j = 0; j < m; ++j) {
+        T *buf = colbuf[j & 1];
+        if (rt.tx == j / RB) {
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const int i = rt.r0 + r;
+                if (i >= j) buf[i] = rt.pick_col(a, r, j % RB);
             }
-            row[c] = v;
         }
         __syncthreads();
-        if (half == 0 && c <= i) s[i * P + c] = row[c];
+        const T d = buf[j];
+        if (!(d > T(0))) return j;          /* uniform: every thread read the same pivot */
+        const T ljj = sqrt(d);
+        const T rinv = T(1) / ljj;
+        T li[RB], lk[RB];
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int i = rt.r0 + r;
+            li[r] = (i > j && i < m) ? buf[i] * rinv : T(0);
+        }
+#pragma unroll
+        for (int c = 0; c < RB; ++c) {
+            const int k = rt.c0 + c;
+            lk[c] = (k > j && k < m) ? buf[k] * rinv : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int i = rt.r0 + r;
+#pragma unroll
+            for (int c = 0; c < RB; ++c) {
+                const int k = rt.c0 + c;
+                if (k == j) {
+                    if (i == j) a[r][c] = ljj;
+                    else if (i > j) a[r][c] = li[r];
+                } else if (k > j && i >= k) {
+                    a[r][c] -= li[r] * lk[c];
+                }
+            }
+        }
+    }
+    return -1;
+}
+
+/*
+ * X = L^{-1} for the m x m lower-triangular L in registers `a`, by
+ * right-looking forward elimination on X = I: step j publishes column j of L
+ * and row j of X, then X(j, :) /= L(j, j) and X(i, :) -= L(i, j) X(j, :) for
+ * i > j.  One barrier per step (double-buffered LDS rows / columns).
+ */
+template <typename T, int NB>
+__device__ void inv_regs(const RegTile<T, NB> &rt, const T (&a)[NB / 16][NB / 16], T (&x)[NB / 16][NB / 16],
+                         T (*colbuf)[NB], T (*rowbuf)[NB], int m)
+{
+    constexpr int RB = NB / 16;
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+        for (int c = 0; c < RB; ++c) x[r][c] = (rt.r0 + r == rt.c0 + c && rt.r0 + r < m) ? T(1) : T(0);
+    for (int j = 0; j < m; ++j) {
+        T *cb = colbuf[j & 1];
+        T *rb = rowbuf[j & 1];
+        if (rt.tx == j / RB) {
+#pragma unroll
+            for (int r = 0; r < RB; ++r) {
+                const int i = rt.r0 + r;
+                if (i >= j) cb[i] = rt.pick_col(a, r, j % RB);
+            }
+        }
+        if (rt.ty == j / RB) {
+#pragma unroll
+            for (int r = 0; r < RB; ++r)
+                if (rt.r0 + r == j) {
+#pragma unroll
+                    for (int c = 0; c < RB; ++c) rb[rt.c0 + c] = x[r][c];
+                }
+        }
         __syncthreads();
+        const T rinv = T(1) / cb[j];
+        T xj[RB], lij[RB];
+#pragma unroll
+        for (int c = 0; c < RB; ++c) xj[c] = rb[rt.c0 + c] * rinv;
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int i = rt.r0 + r;
+            lij[r] = (i > j && i < m) ? cb[i] : T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int i = rt.r0 + r;
+#pragma unroll
+            for (int c = 0; c < RB; ++c) {
+                if (i == j) x[r][c] = xj[c];
+                else if (i > j) x[r][c] -= lij[r] * xj[c];
+            }
+        }
     }
 }
 
@@ -114,55 +227,39 @@ __device__ void write_inverse(const T *s, T *inv, int m, int tid)
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T rsqrt_exact(T x) { return T(1) / sqrt(x); }
-
 template <typename T, int NB>
 __global__ __launch_bounds__(256) void potrf_tile_kernel(T *base, RfpView v, int m, int *info,
                                                          int info_base, T *inv)
 {
     constexpr int P = NB + 1;
+    constexpr int RB = NB / 16;
     __shared__ T s[NB * P];
-    __shared__ T row[NB];
-    __shared__ T part[256];
-    __shared__ int fail;
+    __shared__ T colbuf[2][NB];
+    __shared__ T rowbuf[2][NB];
     const int tid = threadIdx.x;
     if (*info != INT_MAX) return;          /* an earlier block already failed */
 
+    RegTile<T, NB> rt;
+    rt.init(tid);
     load_lower<T, NB, P>(s, base, v, m, tid);
-    if (tid == 0) fail = -1;
     __syncthreads();
-
-    /* right-looking column Cholesky; 16 x 16 thread grid on the update */
-    const int tx = tid & 15, ty = tid >> 4;
-    for (int j = 0; j < m; ++j) {
-        const T d = s[j * P + j];
-        if (!(d > T(0))) {                 /* uniform: every thread read the same d */
-            if (tid == 0) fail = j;
-            break;
-        }
-        const T ljj = sqrt(d);
-        const T rinv = T(1) / ljj;
-        __syncthreads();                   /* all threads have read d */
-        if (tid == 0) s[j * P + j] = ljj;
-        for (int i = j + 1 + tid; i < m; i += 256) s[i * P + j] *= rinv;
-        __syncthreads();
-        for (int i = j + 1 + ty; i < m; i += 16) {
-            const T lij = s[i * P + j];
-            for (int k = j + 1 + tx; k <= i; k += 16) s[i * P + k] -= lij * s[k * P + j];
-        }
-        __syncthreads();
-    }
+    T a[RB][RB];
+    rt.template from_lds<P>(s, a);
+    const int fail = chol_regs<T, NB>(rt, a, colbuf, m);
     __syncthreads();
-
+    rt.template to_lds<P>(s, a);
+    __syncthreads();
     store_lower<T, NB, P>(s, base, v, m, tid);
     if (fail >= 0) {
         if (tid == 0) atomicMin(info, info_base + fail + 1);
         return;
     }
     if (inv) {
+        T x[RB][RB];
+        inv_regs<T, NB>(rt, a, x, colbuf, rowbuf, m);
         __syncthreads();
-        invert_lower<T, NB, P>(s, row, part, m, tid);
+        rt.template to_lds<P>(s, x);
+        __syncthreads();
         write_inverse<T, NB, P>(s, inv, m, tid);
     }
 }
@@ -171,16 +268,24 @@ template <typename T, int NB>
 __global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView v, int64_t m, T *inv)
 {
     constexpr int P = NB + 1;
+    constexpr int RB = NB / 16;
     __shared__ T s[NB * P];
-    __shared__ T row[NB];
-    __shared__ T part[256];
+    __shared__ T colbuf[2][NB];
+    __shared__ T rowbuf[2][NB];
     const int tid = threadIdx.x;
     const int64_t t = blockIdx.x;
     const int64_t r0 = t * NB;
     const int mt = (int)((m - r0) < NB ? (m - r0) : NB);
+    RegTile<T, NB> rt;
+    rt.init(tid);
     load_lower<T, NB, P>(s, base, v.sub(r0, r0), mt, tid);
     __syncthreads();
-    invert_lower<T, NB, P>(s, row, part, mt, tid);
+    T a[RB][RB], x[RB][RB];
+    rt.template from_lds<P>(s, a);
+    inv_regs<T, NB>(rt, a, x, colbuf, rowbuf, mt);
+    __syncthreads();
+    rt.template to_lds<P>(s, x);
+    __syncthreads();
     write_inverse<T, NB, P>(s, inv + t * NB * NB, mt, tid);
 }
 
