@@ -289,13 +289,228 @@ __global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView
     write_inverse<T, NB, P>(s, inv + t * NB * NB, mt, tid);
 }
 
+/* ------------------------------------------------------------------------ */
+/* FP32 leaf: blocked, MFMA                                                  */
+/* ------------------------------------------------------------------------ */
+/*
+ * The 128 x 128 FP32 leaf is factored as 4 x 4 blocks of 32 x 32 held in LDS:
+ * for each block column p, one wave factors the diagonal block (one row per
+ * lane, pivots and L(k, j) broadcast with readlane) and inverts it; the
+ * panel below is solved as L(i, p) := L(i, p) * inv(L(p, p))^T and the
+ * trailing blocks are updated, each as 32 x 32 x 32 v_mfma_f32_32x32x2_f32
+ * products with the four waves taking different blocks.  The inverse of the
+ * whole leaf is then assembled block by block,
+ *     X(i, j) = -X(i, i) * sum_{j <= k < i} L(i, k) X(k, j),
+ * again from MFMA block products.  A ragged leaf (m < 128) is padded with the
+ * identity, which factors and inverts to itself.
+ */
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+namespace leaf32 {
+
+constexpr int NB = 128, BS = 32, NBLK = 4, LP = NB + 1;
+
+/* acc += A * B over a 32 x 32 x 32 block product; element (i, k) of A at
+ * A[i*ai + k*ak], element (k, j) of B at B[k*bk + j*bj] (all in LDS). */
+__device__ __forceinline__ void mma(f32x16_t &acc, const float *A, int ai, int ak, const float *B, int bk, int bj,
+                                    int lane)
+{
+    const int r = lane & 31, h = lane >> 5;
+#pragma unroll
+    for (int k0 = 0; k0 < BS; k0 += 2) {
+        const float a = A[r * ai + (k0 + h) * ak];
+        const float b = B[(k0 + h) * bk + r * bj];
+        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void zero(f32x16_t &acc)
+{
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+}
+
+/* C[i*ci + j] = alpha*acc (+ C when add) for the block held in acc */
+__device__ __forceinline__ void store(const f32x16_t &acc, float *C, int ci, float alpha, bool add, int lane,
+                                      bool lower_only = false)
+{
+    const int j = lane & 31;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int i = 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (lower_only && i < j) continue;
+        float *c = C + i * ci + j;
+        *c = add ? *c + alpha * acc[r] : alpha * acc[r];
+    }
+}
+
+__device__ __forceinline__ float bcast(float v, int lane)
+{
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+/*
+ * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, one row
+ * per lane (lanes 32..63 idle), write L into D's lower triangle and inv(L)
+ * into Xd.  Returns the failing local column or -1 (wave-uniform).
+ */
+__device__ int factor_diag(float *D, float *Xd, int lane)
+{
+    const int i = lane & 31;
+    const bool act = lane < 32;
+    float r[BS];
+#pragma unroll
+    for (int k = 0; k < BS; ++k) r[k] = (act && k <= i) ? D[i * LP + k] : 0.f;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        const float piv = bcast(r[j], j);
+        if (!(piv > 0.f)) return j;
+        const float ljj = sqrtf(piv);
+        const float rinv = 1.f / ljj;
+        r[j] = (i == j) ? ljj : ((i > j) ? r[j] * rinv : r[j]);
+#pragma unroll
+        for (int k = j + 1; k < BS; ++k) {
+            const float lkj = bcast(r[j], k);
+            if (i >= k) r[k] -= r[j] * lkj;
+        }
+    }
+    if (act) {
+#pragma unroll
+        for (int k = 0; k < BS; ++k)
+            if (k <= i) D[i * LP + k] = r[k];
+    }
+    /* inverse, one column per lane: x(c) = e_c solved by forward substitution;
+     * lane c holds column c of X in registers, L entries broadcast from r */
+    float x[BS];
+#pragma unroll
+    for (int q = 0; q < BS; ++q) {
+        /* row q of X:  x[q] = (delta(q, c) - sum_{k<q} L(q, k) x[k]) / L(q, q) */
+        float acc = (q == i) ? 1.f : 0.f;
+#pragma unroll
+        for (int k = 0; k < q; ++k) acc -= bcast(r[k], q) * x[k];
+        x[q] = acc / bcast(r[q], q);
+    }
+    if (act) {
+#pragma unroll
+        for (int q = 0; q < BS; ++q) Xd[q * LP + i] = (q >= i) ? x[q] : 0.f;
+    }
+    return -1;
+}
+
+__global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v, int m, int *info, int info_base,
+                                                         float *inv)
+{
+    __shared__ float L[NB * LP];
+    __shared__ float X[NB * LP];
+    __shared__ float T[4][BS * 33];
+    __shared__ int fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (*info != INT_MAX) return;
+
+    /* load the lower triangle; identity padding beyond m */
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        int i, j;
+        if (v.rs == 1) { i = idx % NB; j = idx / NB; }
+        else           { j = idx % NB; i = idx / NB; }
+        float x = 0.f;
+        if (i < m && j < m) {
+            if (i >= j) x = base[v.off + (int64_t)i * v.rs + (int64_t)j * v.cs];
+        } else if (i == j) {
+            x = 1.f;
+        }
+        L[i * LP + j] = x;
+        X[i * LP + j] = 0.f;
+    }
+    if (tid == 0) fail = -1;
+    __syncthreads();
+
+    for (int p = 0; p < NBLK; ++p) {
+        float *Dpp = L + (p * BS) * LP + p * BS;
+        if (wave == 0) {
+            const int f = factor_diag(Dpp, X + (p * BS) * LP + p * BS, lane);
+            if (lane == 0 && f >= 0) fail = p * BS + f;
+        }
+        __syncthreads();
+        if (fail >= 0) break;
+        /* panel: L(i, p) := L(i, p) * X(p, p)^T for i > p, one wave per block */
+        const float *Xpp = X + (p * BS) * LP + p * BS;
+        for (int i = p + 1 + wave; i < NBLK; i += 4) {
+            float *Lip = L + (i * BS) * LP + p * BS;
+            f32x16_t acc;
+            zero(acc);
+            mma(acc, Lip, LP, 1, Xpp, 1, LP, lane);   /* B(k, j) = X(j, k) */
+            store(acc, Lip, LP, 1.f, false, lane);      /* the wave read all of Lip before writing */
+        }
+        __syncthreads();
+        /* trailing: L(i, j) -= L(i, p) L(j, p)^T for p < j <= i */
+        int blk = 0;
+        for (int i = p + 1; i < NBLK; ++i)
+            for (int j = p + 1; j <= i; ++j, ++blk) {
+                if ((blk & 3) != wave) continue;
+                f32x16_t acc;
+                zero(acc);
+                mma(acc, L + (i * BS) * LP + p * BS, LP, 1, L + (j * BS) * LP + p * BS, 1, LP, lane);
+                store(acc, L + (i * BS) * LP + j * BS, LP, -1.f, true, lane, i == j);
+            }
+        __syncthreads();
+    }
+
+    /* write L back (also after a failure: LAPACK leaves a partial factor) */
+    if (v.rs == 1) {
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int i = idx % NB, j = idx / NB;
+            if (i < m && j < m && i >= j) base[v.off + i + (int64_t)j * v.cs] = L[i * LP + j];
+        }
+    } else {
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int j = idx % NB, i = idx / NB;
+            if (i < m && j < m && i >= j) base[v.off + (int64_t)i * v.rs + j] = L[i * LP + j];
+        }
+    }
+    if (fail >= 0) {
+        if (tid == 0) atomicMin(info, info_base + fail + 1);
+        return;
+    }
+    if (!inv) return;
+
+    /* blocks of the inverse below the diagonal, by distance d = i - j */
+    for (int d = 1; d < NBLK; ++d) {
+        for (int j = wave; j + d < NBLK; j += 4) {
+            const int i = j + d;
+            f32x16_t acc;
+            zero(acc);
+            for (int k = j; k < i; ++k)
+                mma(acc, L + (i * BS) * LP + k * BS, LP, 1, X + (k * BS) * LP + j * BS, LP, 1, lane);
+            store(acc, T[wave], 33, 1.f, false, lane);
+            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            /* X(i, j) = -X(i, i) T */
+            f32x16_t acc2;
+            zero(acc2);
+            mma(acc2, X + (i * BS) * LP + i * BS, LP, 1, T[wave], 33, 1, lane);
+            store(acc2, X + (i * BS) * LP + j * BS, LP, -1.f, false, lane);
+        }
+        __syncthreads();
+    }
+    for (int idx = tid; idx < NB * NB; idx += 256) {
+        const int i = idx / NB, j = idx % NB;
+        inv[idx] = (i < m && j < m && j <= i) ? X[i * LP + j] : 0.f;
+    }
+}
+
+}  // namespace leaf32
+
 template <typename T>
 static hipError_t potrf_tile(T *base, RfpView v, int m, int *info, int info_base, T *inv, hipStream_t s)
 {
     constexpr int NB = LeafSize<T>::value;
     if (m <= 0) return hipSuccess;
     if (m > NB) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((potrf_tile_kernel<T, NB>), dim3(1), dim3(256), 0, s, base, v, m, info, info_base, inv);
+    if (sizeof(T) == 4)
+        hipLaunchKernelGGL(leaf32::potrf_leaf_kernel, dim3(1), dim3(256), 0, s, (float *)base, v, m, info, info_base,
+                           (float *)inv);
+    else
+        hipLaunchKernelGGL((potrf_tile_kernel<T, NB>), dim3(1), dim3(256), 0, s, base, v, m, info, info_base, inv);
     return hipGetLastError();
 }
 
