@@ -75,7 +75,15 @@ struct KParams {
     int lower;
     int64_t kchunk;     /* split-K slice length (== K without split-K) */
     int64_t c_zstride;  /* element stride between the partial C of two slices */
+    unsigned short *sh_hi, *sh_lo;  /* optional BF16 copy of C (hi, lo parts) */
+    int64_t sh_off, sh_rs, sh_cs;
 };
+
+__device__ __forceinline__ unsigned short bf16_rne(float x)
+{
+    __bf16 b = (__bf16)x;
+    return __builtin_bit_cast(unsigned short, b);
+}
 
 /* Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
  * contiguous range of tile ids so neighbouring tiles share its L2. */
@@ -273,6 +281,12 @@ __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
                     T val = alpha * acc[i][j][r];
                     if (use_beta) val += beta * (*cp);
                     *cp = val;
+                    if (p.sh_hi) {
+                        const int64_t so = p.sh_off + gi * p.sh_rs + gj * p.sh_cs;
+                        const unsigned short hb = bf16_rne((float)val);
+                        p.sh_hi[so] = hb;
+                        if (p.sh_lo) p.sh_lo[so] = bf16_rne((float)val - __uint_as_float((unsigned)hb << 16));
+                    }
                 }
             }
         }
@@ -313,6 +327,8 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
     p.tiles_n = (g.N + BN - 1) / BN;
     p.kchunk = g.K;
     p.c_zstride = 0;
+    p.sh_hi = g.c_sh_hi; p.sh_lo = g.c_sh_lo;
+    p.sh_off = g.vcs.off; p.sh_rs = g.vcs.rs; p.sh_cs = g.vcs.cs;
     if (g.lower) {
         if (BM != BN || g.M != g.N) return hipErrorInvalidValue;
         p.ntiles = p.tiles_m * (p.tiles_m + 1) / 2;
@@ -340,6 +356,7 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
         p.c = g.partial; p.c_off = 0; p.c_rs = 1; p.c_cs = g.M;
         p.c_zstride = g.M * g.N;
         p.alpha = 1.0; p.beta = 0.0;
+        p.sh_hi = p.sh_lo = nullptr;
     }
     dim3 grid((unsigned)p.ntiles, (unsigned)splits), block(256);
     if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, true, true>), grid, block, 0, s, p);

@@ -266,12 +266,221 @@ hipError_t launch_typed(const BKParams &p, bool akc, bool bkc, hipStream_t s)
     return hipGetLastError();
 }
 
+/*
+ * BF16-operand kernel for the factorisation's updates.  A and B are BF16
+ * row blocks with k contiguous (the factor's BF16 shadow, see
+ * rfp_drivers.hip): A(i, k) = a[a_off + i*a_ld + k], rows 16-byte aligned.
+ * Tile BT x BT (BT = 256: 4 waves of 128 x 128, 4 x 4 MFMA blocks each, 256
+ * accumulator registers per lane; BT = 128 for split-BF16, whose hi + lo
+ * operands double the LDS), BK = 32, double-buffered LDS, one 16-byte global
+ * load and one 16-byte LDS store per vector, one 16-byte LDS read per MFMA
+ * operand.
+ */
+struct RowParams {
+    const unsigned short *a_hi, *a_lo; int64_t a_off, a_ld;
+    const unsigned short *b_hi, *b_lo; int64_t b_off, b_ld;
+    float *c; int64_t c_off, c_rs, c_cs;
+    int64_t M, N, K;
+    float alpha, beta;
+    int64_t tiles_m, tiles_n, ntiles;
+    int lower;
+};
+
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+template <int BT, int BK>
+struct RowStage {
+    static constexpr int NV = BT * BK / 8 / 256;   /* 16-byte vectors per thread */
+    u16x8 v[NV];
+
+    __device__ __forceinline__ void load(const unsigned short *base, int64_t off, int64_t ld, int64_t r0, int64_t k0,
+                                         int64_t R, int64_t K, int tid)
+    {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int id = tid + 256 * e;
+            const int row = id / (BK / 8), kq = (id % (BK / 8)) * 8;
+            const int64_t gr = r0 + row, gk = k0 + kq;
+            const unsigned short *p = base + off + gr * ld + gk;
+            if (gr < R && gk + 8 <= K) {
+                v[e] = *(const u16x8 *)p;
+            } else {
+                u16x8 t;
+#pragma unroll
+                for (int c = 0; c < 8; ++c) t[c] = (gr < R && gk + c < K) ? p[c] : (unsigned short)0;
+                v[e] = t;
+            }
+        }
+    }
+
+    template <int P>
+    __device__ __forceinline__ void store(unsigned short *s, int tid) const
+    {
+#pragma unroll
+        for (int e = 0; e < NV; ++e) {
+            const int id = tid + 256 * e;
+            const int row = id / (BK / 8), kq = (id % (BK / 8)) * 8;
+            *(u16x8 *)(s + row * P + kq) = v[e];
+        }
+    }
+};
+
+template <int BT, bool X3>
+__global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
+{
+    constexpr int BK = 32, P = BK + 8;
+    constexpr int WT = BT / 2, FM = WT / 32;
+    constexpr int NS = X3 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) unsigned short sA[2][NS][BT * P];
+    __shared__ __attribute__((aligned(16))) unsigned short sB[2][NS][BT * P];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    int64_t tm, tn;
+    {
+        BKParams q;
+        q.tiles_m = p.tiles_m; q.tiles_n = p.tiles_n; q.lower = p.lower;
+        tile_coords(q, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
+    }
+    const int64_t m0 = tm * BT, n0 = tn * BT;
+
+    f32x16 acc[FM][FM];
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FM; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    RowStage<BT, BK> ah, bh, al, bl;
+    const int64_t nk = (p.K + BK - 1) / BK;
+    if (nk > 0) {
+        ah.load(p.a_hi, p.a_off, p.a_ld, m0, 0, p.M, p.K, tid);
+        bh.load(p.b_hi, p.b_off, p.b_ld, n0, 0, p.N, p.K, tid);
+        if (X3) {
+            al.load(p.a_lo, p.a_off, p.a_ld, m0, 0, p.M, p.K, tid);
+            bl.load(p.b_lo, p.b_off, p.b_ld, n0, 0, p.N, p.K, tid);
+        }
+        ah.template store<P>(sA[0][0], tid);
+        bh.template store<P>(sB[0][0], tid);
+        if (X3) {
+            al.template store<P>(sA[0][NS - 1], tid);
+            bl.template store<P>(sB[0][NS - 1], tid);
+        }
+    }
+    __syncthreads();
+
+    const int lrow = lane & 31, lk = (lane >> 5) * 8;
+    for (int64_t kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            const int64_t k0 = (kt + 1) * BK;
+            /* the next slice's loads stay in flight during this slice's MFMAs */
+            ah.load(p.a_hi, p.a_off, p.a_ld, m0, k0, p.M, p.K, tid);
+            bh.load(p.b_hi, p.b_off, p.b_ld, n0, k0, p.N, p.K, tid);
+            if (X3) {
+                al.load(p.a_lo, p.a_off, p.a_ld, m0, k0, p.M, p.K, tid);
+                bl.load(p.b_lo, p.b_off, p.b_ld, n0, k0, p.N, p.K, tid);
+            }
+        }
+#pragma unroll 1
+        for (int k = 0; k < BK; k += 16) {
+            bf16x8 fa[FM], fb[FM], fal[FM], fbl[FM];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) {
+                const int off = (wm * WT + i * 32 + lrow) * P + k + lk;
+                fa[i] = *(const bf16x8 *)(sA[cur][0] + off);
+                if (X3) fal[i] = *(const bf16x8 *)(sA[cur][NS - 1] + off);
+            }
+#pragma unroll
+            for (int j = 0; j < FM; ++j) {
+                const int off = (wn * WT + j * 32 + lrow) * P + k + lk;
+                fb[j] = *(const bf16x8 *)(sB[cur][0] + off);
+                if (X3) fbl[j] = *(const bf16x8 *)(sB[cur][NS - 1] + off);
+            }
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FM; ++j) {
+                    if (X3) {
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fb[j], acc[i][j], 0, 0, 0);
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fbl[j], acc[i][j], 0, 0, 0);
+                    }
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+                }
+        }
+        if (more) {
+            ah.template store<P>(sA[cur ^ 1][0], tid);
+            bh.template store<P>(sB[cur ^ 1][0], tid);
+            if (X3) {
+                al.template store<P>(sA[cur ^ 1][NS - 1], tid);
+                bl.template store<P>(sB[cur ^ 1][NS - 1], tid);
+            }
+        }
+        __syncthreads();
+    }
+
+    const bool use_beta = p.beta != 0.f;
+#pragma unroll
+    for (int i = 0; i < FM; ++i)
+#pragma unroll
+        for (int j = 0; j < FM; ++j) {
+            const int64_t gj = n0 + wn * WT + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * WT + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
+                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
+                    float val = p.alpha * acc[i][j][r];
+                    if (use_beta) val += p.beta * (*cp);
+                    *cp = val;
+                }
+            }
+        }
+}
+
+template <int BT, bool X3>
+hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
+{
+    RowParams p;
+    p.a_hi = (const unsigned short *)g.a; p.a_lo = (const unsigned short *)g.a_lo;
+    p.a_off = g.va.off; p.a_ld = g.va.rs;
+    p.b_hi = (const unsigned short *)g.b; p.b_lo = (const unsigned short *)g.b_lo;
+    p.b_off = g.vb.off; p.b_ld = g.vb.rs;
+    p.c = (float *)g.c; p.c_off = g.vc.off; p.c_rs = g.vc.rs; p.c_cs = g.vc.cs;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.alpha = (float)g.alpha; p.beta = (float)g.beta;
+    p.lower = g.lower ? 1 : 0;
+    p.tiles_m = (g.M + BT - 1) / BT;
+    p.tiles_n = (g.N + BT - 1) / BT;
+    p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
+    if (p.ntiles <= 0) return hipSuccess;
+    if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     if (g.inplace || g.splits > 1) return hipErrorInvalidValue;
+    if (g.bf16_rows) {
+        /* BF16 row blocks (k contiguous, 16-byte aligned rows) */
+        if (g.va.cs != 1 || g.vb.cs != 1 || (g.va.rs % 8) || (g.vb.rs % 8) || (g.va.off % 8) || (g.vb.off % 8))
+            return hipErrorInvalidValue;
+        if (g.lower && g.M != g.N) return hipErrorInvalidValue;
+        const bool x3 = math == RFP_MATH_BF16X3;
+        if (x3 && (!g.a_lo || !g.b_lo)) return hipErrorInvalidValue;
+        const bool big = g.M >= 2048 && g.N >= 2048;
+        if (x3) return launch_rows<128, true>(g, s);
+        return big ? launch_rows<256, false>(g, s) : launch_rows<128, false>(g, s);
+    }
     BKParams p;
     p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;
     p.b = g.b; p.b_off = g.vb.off; p.b_rs = g.vb.rs; p.b_cs = g.vb.cs;

@@ -64,6 +64,8 @@ extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
     ctx->ws_bytes = 0;
     ctx->d_info = nullptr;
     ctx->prof = false;
+    ctx->sh = nullptr;
+    ctx->sh_bytes = 0;
     for (RfpKernelStat &st : ctx->prof_stats) st = RfpKernelStat{0, 0.0, 0.0, 0.0};
     if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return CURFP_STATUS_NOT_INITIALIZED; }
     if (hipMalloc((void **)&ctx->d_info, sizeof(int)) != hipSuccess) { delete ctx; return CURFP_STATUS_ALLOC_FAILED; }
@@ -81,6 +83,7 @@ extern "C" curfpStatus_t curfpDestroy(curfpHandle_t handle)
     }
     for (hipEvent_t e : handle->prof_pool) (void)hipEventDestroy(e);
     if (handle->ws) (void)hipFree(handle->ws);
+    if (handle->sh) (void)hipFree(handle->sh);
     if (handle->d_info) (void)hipFree(handle->d_info);
     delete handle;
     return CURFP_STATUS_SUCCESS;
@@ -330,6 +333,16 @@ static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
 /* spftrf                                                                    */
 /* ------------------------------------------------------------------------ */
 
+/*
+ * In BF16 / split-BF16 mode the factorisation keeps a BF16 shadow of the
+ * factor: three dense row-major blocks (L11, L21, L22; leading dimensions
+ * rounded up to 8 elements so rows are 16-byte aligned) holding bf16(L) (and
+ * bf16(L - bf16(L)) for split-BF16).  A block of L is final when a leaf
+ * triangular solve writes it; that GEMM's epilogue writes the shadow too.
+ * Every BF16 update (the TRSM updates and the SYRKs) reads only final
+ * blocks, so it reads its operands from the shadow: 2 bytes per element, k
+ * contiguous, no conversion in the hot loop.
+ */
 template <typename T>
 struct Factor {
     curfpHandle_t h;
@@ -337,9 +350,11 @@ struct Factor {
     T *inv;        /* leaf inverse slots */
     int64_t leaf;
     RfpMath math;  /* arithmetic of the SYRK / GEMM updates */
+    unsigned short *sh_hi, *sh_lo;  /* BF16 shadow (null in FP32 / FP64 mode) */
 
     hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t N,
-                    int64_t K, double alpha, double beta, bool lower, int inplace, RfpMath m)
+                    int64_t K, double alpha, double beta, bool lower, int inplace, RfpMath m,
+                    const RfpView *shadow = nullptr)
     {
         GemmArgs g = gemm_args();
         g.a = x; g.va = vx;
@@ -349,34 +364,58 @@ struct Factor {
         g.alpha = alpha; g.beta = beta;
         g.lower = lower;
         g.inplace = inplace;
+        if (shadow && sh_hi) {
+            g.c_sh_hi = sh_hi;
+            g.c_sh_lo = sh_lo;
+            g.vcs = *shadow;
+        }
         return run_gemm<T>(h, g, m);
     }
 
+    /* C -= X Y^T with X, Y read from the BF16 shadow */
+    hipError_t gemm_shadow(RfpView c, RfpView sx, RfpView sy, int64_t M, int64_t N, int64_t K, bool lower)
+    {
+        GemmArgs g = gemm_args();
+        g.a = sh_hi; g.va = sx; g.a_lo = sh_lo;
+        g.b = sh_hi; g.vb = sy; g.b_lo = sh_lo;
+        g.bf16_rows = true;
+        g.c = a; g.vc = c;
+        g.M = M; g.N = N; g.K = K;
+        g.alpha = -1.0; g.beta = 1.0;
+        g.lower = lower;
+        return run_gemm<T>(h, g, math);
+    }
+
     /* B (mb x m) := B * L^{-T}, L = lower-triangular view (m x m) whose
-     * leaf inverses start at `slot` */
-    hipError_t trsm_rt(RfpView B, int64_t mb, RfpView L, int64_t m, int64_t slot)
+     * leaf inverses start at `slot`; Bs / Ls are the shadow views */
+    hipError_t trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return hipSuccess;
         if (m <= leaf)
             return gemm(B, a, B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, mb, m, m, 1.0, 0.0,
-                        false, 1, RFP_MATH_F32);
+                        false, 1, RFP_MATH_F32, &Bs);
         const int64_t m1 = leaf_split(m, leaf);
-        RFP_TRY(trsm_rt(B, mb, L, m1, slot));
-        RFP_TRY(gemm(B.sub(0, m1), a, B, a, L.sub(m1, 0), mb, m - m1, m1, -1.0, 1.0, false, 0, math));
-        return trsm_rt(B.sub(0, m1), mb, L.sub(m1, m1), m - m1, slot + m1 / leaf);
+        RFP_TRY(trsm_rt(B, Bs, mb, L, Ls, m1, slot));
+        if (sh_hi)
+            RFP_TRY(gemm_shadow(B.sub(0, m1), Bs, Ls.sub(m1, 0), mb, m - m1, m1, false));
+        else
+            RFP_TRY(gemm(B.sub(0, m1), a, B, a, L.sub(m1, 0), mb, m - m1, m1, -1.0, 1.0, false, 0, math));
+        return trsm_rt(B.sub(0, m1), Bs.sub(0, m1), mb, L.sub(m1, m1), Ls.sub(m1, m1), m - m1, slot + m1 / leaf);
     }
 
-    hipError_t potrf2(RfpView V11, RfpView V21, RfpView V22, int64_t m1, int64_t m2, int64_t goff,
-                      int64_t slot1, int64_t slot2)
+    hipError_t potrf2(RfpView V11, RfpView S11, RfpView V21, RfpView S21, RfpView V22, RfpView S22, int64_t m1,
+                      int64_t m2, int64_t goff, int64_t slot1, int64_t slot2)
     {
-        RFP_TRY(potrf(V11, m1, goff, slot1));
-        RFP_TRY(trsm_rt(V21, m2, V11, m1, slot1));
-        if (m1 > 0 && m2 > 0)
-            RFP_TRY(gemm(V22, a, V21, a, V21, m2, m2, m1, -1.0, 1.0, true, 0, math));
-        return potrf(V22, m2, goff + m1, slot2);
+        RFP_TRY(potrf(V11, S11, m1, goff, slot1));
+        RFP_TRY(trsm_rt(V21, S21, m2, V11, S11, m1, slot1));
+        if (m1 > 0 && m2 > 0) {
+            if (sh_hi) RFP_TRY(gemm_shadow(V22, S21, S21, m2, m2, m1, true));
+            else RFP_TRY(gemm(V22, a, V21, a, V21, m2, m2, m1, -1.0, 1.0, true, 0, math));
+        }
+        return potrf(V22, S22, m2, goff + m1, slot2);
     }
 
-    hipError_t potrf(RfpView V, int64_t m, int64_t goff, int64_t slot)
+    hipError_t potrf(RfpView V, RfpView S, int64_t m, int64_t goff, int64_t slot)
     {
         if (m <= 0) return hipSuccess;
         if (m <= leaf)
@@ -384,9 +423,40 @@ struct Factor {
                 return Prec<T>::potrf_tile(a, V, (int)m, h->d_info, (int)goff, inv + slot * leaf * leaf, h->stream);
             });
         const int64_t m1 = leaf_split(m, leaf);
-        return potrf2(V, V.sub(m1, 0), V.sub(m1, m1), m1, m - m1, goff, slot, slot + m1 / leaf);
+        return potrf2(V, S, V.sub(m1, 0), S.sub(m1, 0), V.sub(m1, m1), S.sub(m1, m1), m1, m - m1, goff, slot,
+                      slot + m1 / leaf);
     }
 };
+
+/* BF16 shadow geometry: L11 (n1 x n1, ld1), L21 (n2 x n1, ld1), L22 (n2 x n2, ld2) */
+struct ShadowGeom {
+    RfpView s11, s21, s22;
+    int64_t elems;
+};
+static ShadowGeom shadow_geom(const RfpGeom &g)
+{
+    const int64_t ld1 = (g.n1 + 7) / 8 * 8, ld2 = (g.n2 + 7) / 8 * 8;
+    ShadowGeom sg;
+    sg.s11 = RfpView{0, ld1, 1};
+    sg.s21 = RfpView{g.n1 * ld1, ld1, 1};
+    sg.s22 = RfpView{(g.n1 + g.n2) * ld1, ld2, 1};
+    sg.elems = (g.n1 + g.n2) * ld1 + g.n2 * ld2;
+    return sg;
+}
+
+hipError_t rfp_shadow(curfpHandle_t h, size_t bytes, void **out)
+{
+    if (bytes == 0) bytes = 256;
+    if (h->sh_bytes < bytes) {
+        RFP_TRY(hipStreamSynchronize(h->stream));
+        if (h->sh) { (void)hipFree(h->sh); h->sh = nullptr; h->sh_bytes = 0; }
+        const size_t sz = (bytes + (1u << 20) - 1) & ~((size_t)(1u << 20) - 1);
+        RFP_TRY(hipMalloc(&h->sh, sz));
+        h->sh_bytes = sz;
+    }
+    *out = h->sh;
+    return hipSuccess;
+}
 
 template <typename T>
 static curfpStatus_t pftrf_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
@@ -408,8 +478,17 @@ static curfpStatus_t pftrf_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     if (e != hipSuccess) return st_of(e);
     e = launch_set_int(h->d_info, INT_MAX, h->stream);
     if (e != hipSuccess) return st_of(e);
-    Factor<T> f{h, A, (T *)ws, leaf, Prec<T>::f64 ? RFP_MATH_F64 : math_of(h)};
-    e = f.potrf2(g.v11, g.v21, g.v22, g.n1, g.n2, 0, 0, s1);
+    Factor<T> f{h, A, (T *)ws, leaf, Prec<T>::f64 ? RFP_MATH_F64 : math_of(h), nullptr, nullptr};
+    const ShadowGeom sg = shadow_geom(g);
+    if (!Prec<T>::f64 && f.math != RFP_MATH_F32) {
+        const bool x3 = f.math == RFP_MATH_BF16X3;
+        void *sh = nullptr;
+        e = rfp_shadow(h, (size_t)sg.elems * 2 * (x3 ? 2 : 1), &sh);
+        if (e != hipSuccess) return st_of(e);
+        f.sh_hi = (unsigned short *)sh;
+        f.sh_lo = x3 ? f.sh_hi + sg.elems : nullptr;
+    }
+    e = f.potrf2(g.v11, sg.s11, g.v21, sg.s21, g.v22, sg.s22, g.n1, g.n2, 0, 0, s1);
     if (e != hipSuccess) return st_of(e);
     int hinfo = INT_MAX;
     e = hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream);

@@ -113,6 +113,8 @@ struct curfpContext {
     curfpMathMode_t math;
     void           *ws;        /* cached device workspace (grown on demand) */
     size_t          ws_bytes;
+    void           *sh;        /* cached BF16 shadow of the factor (BF16 math modes) */
+    size_t          sh_bytes;
     int            *d_info;    /* device int: first failing minor (INT_MAX = none) */
     /* kernel profiling (curfpSetProfiling / curfpGetKernelStats) */
     bool            prof;
@@ -154,6 +156,14 @@ struct GemmArgs {
     bool b_f32;        /* ... and whose B is stored as FP32 too (requires a_f32) */
     int  splits;       /* split-K slices (needs `partial`: splits*M*N elements) */
     void *partial;
+    /* BF16 row-block operands (the factor's BF16 shadow): a / b are BF16 with
+     * cs == 1; a_lo / b_lo are the split-BF16 low parts (BF16X3 only) */
+    bool bf16_rows;
+    const void *a_lo, *b_lo;
+    /* FP32 GEMM epilogue also writes bf16(C) (and the low part) to these BF16
+     * row blocks at view vcs — the shadow of a block that has become final */
+    unsigned short *c_sh_hi, *c_sh_lo;
+    RfpView vcs;
 };
 static inline GemmArgs gemm_args() { GemmArgs g{}; g.splits = 1; return g; }
 hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s);
