@@ -95,6 +95,7 @@ PYBIND11_MODULE(_curfp_C, m)
     kc["pack"] = (int)CURFP_KERNEL_PACK;
     kc["diag"] = (int)CURFP_KERNEL_DIAG;
     kc["other"] = (int)CURFP_KERNEL_OTHER;
+    kc["trsv"] = (int)CURFP_KERNEL_TRSV;
     m.attr("KERNEL_CLASSES") = kc;
 
     /* ---- FP32 ---- */

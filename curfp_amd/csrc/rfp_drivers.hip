@@ -523,13 +523,28 @@ struct Solver {
         return run_gemm<T>(h, g, RFP_MATH_F32);
     }
 
+    /* Size of the diagonal blocks solved in one kernel (FP32: 512-row block
+     * solves over 128-row leaf inverses; FP64: single leaves). */
+    int64_t block() const { return Prec<T>::f64 ? leaf : (int64_t)block_solve_rows(); }
+
+    hipError_t leaf_solve(RfpView L, int64_t m, RfpView B, int64_t slot, bool trans)
+    {
+        if (!Prec<T>::f64) {
+            return timed(h, CURFP_KERNEL_TRSV, 2.0 * (double)m * m * nrhs / 2.0, 0.0, [&] {
+                return launch_block_solve_f32((const float *)a, L, (int)m, (const float *)(inv + slot * leaf * leaf),
+                                              (float *)b, B, (int)nrhs, trans, h->stream);
+            });
+        }
+        const RfpView w = trans ? RfpView{0, 1, leaf} : RfpView{0, leaf, 1};
+        return gemm(B, inv + slot * leaf * leaf, w, b, B.t(), m, m, 1.0, 0.0, 2);
+    }
+
     /* L Y = B in place, L lower (m x m), B (m x nrhs) */
     hipError_t lower(RfpView L, int64_t m, RfpView B, int64_t slot)
     {
         if (m <= 0) return hipSuccess;
-        if (m <= leaf)
-            return gemm(B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, b, B.t(), m, m, 1.0, 0.0, 2);
-        const int64_t m1 = leaf_split(m, leaf);
+        if (m <= block()) return leaf_solve(L, m, B, slot, false);
+        const int64_t m1 = leaf_split(m, block());
         RFP_TRY(lower(L, m1, B, slot));
         RFP_TRY(gemm(B.sub(m1, 0), a, L.sub(m1, 0), b, B.t(), m - m1, m1, -1.0, 1.0, 0));
         return lower(L.sub(m1, m1), m - m1, B.sub(m1, 0), slot + m1 / leaf);
@@ -539,9 +554,8 @@ struct Solver {
     hipError_t lower_t(RfpView L, int64_t m, RfpView B, int64_t slot)
     {
         if (m <= 0) return hipSuccess;
-        if (m <= leaf)
-            return gemm(B, inv + slot * leaf * leaf, RfpView{0, 1, leaf}, b, B.t(), m, m, 1.0, 0.0, 2);
-        const int64_t m1 = leaf_split(m, leaf);
+        if (m <= block()) return leaf_solve(L, m, B, slot, true);
+        const int64_t m1 = leaf_split(m, block());
         RFP_TRY(lower_t(L.sub(m1, m1), m - m1, B.sub(m1, 0), slot + m1 / leaf));
         RFP_TRY(gemm(B, a, L.sub(m1, 0).t(), b, B.sub(m1, 0).t(), m1, m - m1, -1.0, 1.0, 0));
         return lower_t(L, m1, B, slot);

@@ -195,6 +195,16 @@ hipError_t launch_potrf_tile_f64(double *base, RfpView v, int m, int *info, int 
 hipError_t launch_trtri_tiles_f32(const float *base, RfpView v, int64_t m, float *inv, hipStream_t s);
 hipError_t launch_trtri_tiles_f64(const double *base, RfpView v, int64_t m, double *inv, hipStream_t s);
 
+/*
+ * FP32 block triangular solve (trsv_block.hip): B (m x nrhs view) :=
+ * L^{-1} B (trans = false) or L^{-T} B (trans = true) for the m x m lower
+ * block L (m <= block_solve_rows()), given the inverses of its 128-row leaves
+ * at inv (leaf q at inv + q*128*128, row-major).
+ */
+int block_solve_rows();
+hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const float *inv, float *Bbase, RfpView Bv,
+                                  int nrhs, bool trans, hipStream_t s);
+
 /* RFP <-> dense row-major triangle copies (all 8 variants). */
 hipError_t launch_trttf_f32(const RfpGeom &g, bool lower, const float *A, int64_t lda, float *arf, hipStream_t s);
 hipError_t launch_trttf_f64(const RfpGeom &g, bool lower, const double *A, int64_t lda, double *arf, hipStream_t s);

@@ -175,7 +175,8 @@ typedef enum {
     CURFP_KERNEL_PACK       = 5,   /* strttf / stfttr                     */
     CURFP_KERNEL_DIAG       = 6,   /* add_to_diagonal                     */
     CURFP_KERNEL_OTHER      = 7,   /* fills, reductions, residual setup   */
-    CURFP_KERNEL_CLASSES    = 8
+    CURFP_KERNEL_TRSV       = 8,   /* spftrs diagonal-block solves        */
+    CURFP_KERNEL_CLASSES    = 9
 } curfpKernelClass_t;
 
 curfpStatus_t curfpSetProfiling(curfpHandle_t handle, int enable);
