@@ -23,25 +23,34 @@ template <> struct LeafSize<double> { static constexpr int value = 64; };
 
 int rfp_leaf(bool f64) { return f64 ? LeafSize<double>::value : LeafSize<float>::value; }
 
-/* Load the lower triangle of an m x m view into s (pitch P), zero above. */
+/* Load the lower triangle of an m x m view into s (pitch P), zero above;
+ * all of a thread's loads are issued before its LDS stores. */
 template <typename T, int NB, int P>
 __device__ __forceinline__ void load_lower(T *s, const T *base, RfpView v, int m, int tid)
 {
-    const int total = NB * NB;
+    constexpr int PER = NB * NB / 256;
+    constexpr int STEP = 256 / NB;   /* rows (columns) between a thread's elements */
+    T x[PER];
     if (v.rs == 1) {
-        for (int idx = tid; idx < total; idx += 256) {
-            const int i = idx % NB, j = idx / NB;
-            T x = 0;
-            if (i < m && j < m && i >= j) x = base[v.off + i + (int64_t)j * v.cs];
-            s[i * P + j] = x;
+        const int i = tid % NB, j0 = tid / NB;
+        const T *p = base + v.off + i + (int64_t)j0 * v.cs;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int j = j0 + STEP * e;
+            x[e] = (i < m && j < m && i >= j) ? p[(int64_t)STEP * e * v.cs] : T(0);
         }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) s[i * P + j0 + STEP * e] = x[e];
     } else {
-        for (int idx = tid; idx < total; idx += 256) {
-            const int j = idx % NB, i = idx / NB;
-            T x = 0;
-            if (i < m && j < m && i >= j) x = base[v.off + (int64_t)i * v.rs + j * v.cs];
-            s[i * P + j] = x;
+        const int j = tid % NB, i0 = tid / NB;
+        const T *p = base + v.off + (int64_t)i0 * v.rs + j;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int i = i0 + STEP * e;
+            x[e] = (i < m && j < m && i >= j) ? p[(int64_t)STEP * e * v.rs] : T(0);
         }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) s[(i0 + STEP * e) * P + j] = x[e];
     }
 }
 
@@ -407,19 +416,31 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     if (*info != INT_MAX) return;
 
-    /* load the lower triangle; identity padding beyond m */
-    for (int idx = tid; idx < NB * NB; idx += 256) {
-        int i, j;
-        if (v.rs == 1) { i = idx % NB; j = idx / NB; }
-        else           { j = idx % NB; i = idx / NB; }
-        float x = 0.f;
-        if (i < m && j < m) {
-            if (i >= j) x = base[v.off + (int64_t)i * v.rs + (int64_t)j * v.cs];
-        } else if (i == j) {
-            x = 1.f;
+    /* load the lower triangle (all loads in flight before the LDS stores);
+     * identity padding beyond m */
+    {
+        constexpr int PER = NB * NB / 256;
+        const bool colv = v.rs == 1;
+        float x[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e;
+            const int i = colv ? idx % NB : idx / NB, j = colv ? idx / NB : idx % NB;
+            float t = 0.f;
+            if (i < m && j < m) {
+                if (i >= j) t = base[v.off + (int64_t)i * v.rs + (int64_t)j * v.cs];
+            } else if (i == j) {
+                t = 1.f;
+            }
+            x[e] = t;
         }
-        L[i * LP + j] = x;
-        X[i * LP + j] = 0.f;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e;
+            const int i = colv ? idx % NB : idx / NB, j = colv ? idx / NB : idx % NB;
+            L[i * LP + j] = x[e];
+            X[i * LP + j] = 0.f;
+        }
     }
     if (tid == 0) fail = -1;
     __syncthreads();

@@ -34,16 +34,27 @@ constexpr int BP = CW + 1;     /* LDS pitch of the B slice */
 __device__ __forceinline__ void stage(float *As, const float *base, int64_t off, int64_t rs, int64_t cs, int valid_i,
                                       int valid_k, int tid)
 {
+    constexpr int PER = LF * LF / 256;   /* 64 elements per thread, two rows / columns apart */
+    float v[PER];
+    /* all loads in flight before the first LDS store */
     if (cs == 1) {
-        for (int idx = tid; idx < LF * LF; idx += 256) {
-            const int i = idx / LF, k = idx % LF;
-            As[i * AP + k] = (i < valid_i && k < valid_k) ? base[off + (int64_t)i * rs + k] : 0.f;
-        }
+        const int k = tid % LF, i0 = tid / LF;
+        const float *p = base + off + (int64_t)i0 * rs + k;
+        const int64_t step = 2 * rs;
+        const bool kv = k < valid_k;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[e] = (kv && i0 + 2 * e < valid_i) ? p[e * step] : 0.f;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) As[(i0 + 2 * e) * AP + k] = v[e];
     } else {
-        for (int idx = tid; idx < LF * LF; idx += 256) {
-            const int k = idx / LF, i = idx % LF;
-            As[i * AP + k] = (i < valid_i && k < valid_k) ? base[off + i + (int64_t)k * cs] : 0.f;
-        }
+        const int i = tid % LF, k0 = tid / LF;
+        const float *p = base + off + i + (int64_t)k0 * cs;
+        const int64_t step = 2 * cs;
+        const bool iv = i < valid_i;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) v[e] = (iv && k0 + 2 * e < valid_k) ? p[e * step] : 0.f;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) As[i * AP + k0 + 2 * e] = v[e];
     }
 }
 
@@ -97,11 +108,24 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
     const int cols = min(CW, nrhs - c0);
 
     /* B slice: rows [0, m), columns [c0, c0 + cols) */
-    for (int idx = tid; idx < MAXQ * LF * CW; idx += 256) {
-        int i, c;
-        if (Bv.rs == 1) { i = idx % (MAXQ * LF); c = idx / (MAXQ * LF); }
-        else            { c = idx % CW; i = idx / CW; }
-        Bs[i * BP + c] = (i < m && c < cols) ? Bbase[Bv.off + (int64_t)i * Bv.rs + (int64_t)(c0 + c) * Bv.cs] : 0.f;
+    {
+        constexpr int PER = MAXQ * LF * CW / 256;
+        float v[PER];
+        const bool col_major = Bv.rs == 1;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e;
+            const int i = col_major ? idx % (MAXQ * LF) : idx / CW;
+            const int c = col_major ? idx / (MAXQ * LF) : idx % CW;
+            v[e] = (i < m && c < cols) ? Bbase[Bv.off + (int64_t)i * Bv.rs + (int64_t)(c0 + c) * Bv.cs] : 0.f;
+        }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int idx = tid + 256 * e;
+            const int i = col_major ? idx % (MAXQ * LF) : idx / CW;
+            const int c = col_major ? idx / (MAXQ * LF) : idx % CW;
+            Bs[i * BP + c] = v[e];
+        }
     }
     __syncthreads();
 
