@@ -325,10 +325,10 @@ struct RowStage {
     }
 };
 
-template <int BT, bool X3>
+template <int BT, bool X3, int BK>
 __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
 {
-    constexpr int BK = 32, P = BK + 8;
+    constexpr int P = BK + 8;
     constexpr int WT = BT / 2, FM = WT / 32;
     constexpr int NS = X3 ? 2 : 1;
     __shared__ __attribute__((aligned(16))) unsigned short sA[2][NS][BT * P];
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
         }
 }
 
-template <int BT, bool X3>
+template <int BT, bool X3, int BK>
 hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
 {
     RowParams p;
@@ -460,7 +460,7 @@ hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
     if (p.ntiles <= 0) return hipSuccess;
     if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -478,8 +478,8 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
         const bool x3 = math == RFP_MATH_BF16X3;
         if (x3 && (!g.a_lo || !g.b_lo)) return hipErrorInvalidValue;
         const bool big = g.M >= 2048 && g.N >= 2048;
-        if (x3) return launch_rows<128, true>(g, s);
-        return big ? launch_rows<256, false>(g, s) : launch_rows<128, false>(g, s);
+        if (x3) return launch_rows<128, true, 32>(g, s);
+        return big ? launch_rows<256, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
     }
     BKParams p;
     p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;

@@ -510,6 +510,8 @@ struct Solver {
     T *inv;
     int64_t leaf, nrhs;
 
+    T *partial;          /* split-K scratch, solve_partial_elems() elements */
+
     hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t K,
                     double alpha, double beta, int inplace)
     {
@@ -520,8 +522,22 @@ struct Solver {
         g.M = M; g.N = nrhs; g.K = K;
         g.alpha = alpha; g.beta = beta;
         g.inplace = inplace;
+        if (!inplace && partial) {
+            /* the updates are tall-and-skinny (N = nrhs) with long K: split K
+             * until there are ~4 workgroups per CU */
+            const int64_t tiles = ((M + 127) / 128) * ((nrhs + 63) / 64);
+            int64_t sp = (1024 + tiles - 1) / tiles;
+            sp = std::min<int64_t>(sp, std::min<int64_t>(32, K / 512));
+            while (sp > 1 && sp * M * nrhs > solve_partial_elems()) --sp;
+            if (sp > 1) {
+                g.splits = (int)sp;
+                g.partial = partial;
+            }
+        }
         return run_gemm<T>(h, g, RFP_MATH_F32);
     }
+
+    static int64_t solve_partial_elems() { return (int64_t)8 << 20; }
 
     /* Size of the diagonal blocks solved in one kernel (FP32: 512-row block
      * solves over 128-row leaf inverses; FP64: single leaves). */
@@ -562,11 +578,16 @@ struct Solver {
     }
 };
 
-static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
+static size_t pftrs_inv_elems(const RfpGeom &g, bool f64)
 {
     const int64_t leaf = rfp_leaf(f64);
     const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
-    return (size_t)(s1 + s2) * leaf * leaf * (f64 ? sizeof(double) : sizeof(float));
+    return (size_t)(s1 + s2) * leaf * leaf;
+}
+
+static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
+{
+    return (pftrs_inv_elems(g, f64) + (size_t)(8 << 20)) * (f64 ? sizeof(double) : sizeof(float));
 }
 
 /* Forward + backward substitution on B (view bv, n x nrhs) with the RFP
@@ -580,7 +601,7 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
                   [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
     RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
                   [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
-    Solver<T> sv{h, A, B, inv, leaf, nrhs};
+    Solver<T> sv{h, A, B, inv, leaf, nrhs, inv + pftrs_inv_elems(g, Prec<T>::f64)};
     const RfpView b2 = bv.sub(g.n1, 0);
     /* forward: L y = b */
     RFP_TRY(sv.lower(g.v11, g.n1, bv, 0));
