@@ -98,12 +98,7 @@ __device__ __forceinline__ int64_t xcd_remap(int64_t pid, int64_t total)
 __device__ __forceinline__ void tile_coords(const KParams &p, int64_t t, int64_t &tm, int64_t &tn)
 {
     if (p.lower) {
-        /* t -> (ti, tj), ti >= tj, row-major over the lower triangle of tiles */
-        int64_t ti = (int64_t)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
-        while (ti * (ti + 1) / 2 > t) --ti;
-        while ((ti + 1) * (ti + 2) / 2 <= t) ++ti;
-        tm = ti;
-        tn = t - ti * (ti + 1) / 2;
+        tri_tile_coords(t, p.tiles_m, tm, tn);
     } else {
         /* grouped ordering: GROUP tile-rows walk the tile-columns together */
         const int64_t GROUP = 8;

@@ -306,7 +306,10 @@ static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
     /* row i of op(A) (n x k): trans = N -> A[i + p*lda], trans = T -> A[p + i*lda] */
     const RfpView av = (trans == CURFP_OP_N) ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
     const RfpView a1 = av, a2 = av.sub(g.n1, 0);
-    const RfpMath m = a_bf16 ? (h->math == CURFP_MATH_FP32 ? RFP_MATH_BF16 : math_of(h)) : math_of(h);
+    /* a BF16 input is exact in BF16: its split-BF16 low part is zero */
+    const RfpMath m = a_bf16 ? RFP_MATH_BF16 : math_of(h);
+    /* BF16 rows with k contiguous and 16-byte aligned go to the row-block kernel */
+    const bool rows = a_bf16 && trans == CURFP_OP_T && lda % 8 == 0 && ((uintptr_t)A % 16) == 0;
 
     struct Blk { RfpView c, x, y; int64_t M, N; bool lower; } blk[3] = {
         {g.v11, a1, a1, g.n1, g.n1, true},
@@ -323,6 +326,7 @@ static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
         ga.alpha = (double)a; ga.beta = (double)b;
         ga.lower = bk.lower;
         ga.a_bf16 = a_bf16;
+        ga.bf16_rows = rows;
         hipError_t e = run_gemm<T>(h, ga, m);
         if (e != hipSuccess) return st_of(e);
     }

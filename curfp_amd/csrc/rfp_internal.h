@@ -92,6 +92,39 @@ static inline RfpGeom rfp_geom(int64_t n, bool transr_n, bool lower)
     return g;
 }
 
+/*
+ * Lower-triangular tile order used by every SYRK-shaped launch: tile-rows are
+ * taken in bands of RFP_TILE_BAND; inside a band the tiles are visited column
+ * by column, so the workgroups that run together on one XCD (consecutive ids
+ * after the XCD remap) form a band-high block sharing a few row panels and a
+ * few column panels in L2, instead of one row panel and many column panels.
+ * Maps t in [0, T(T+1)/2) to (ti, tj), ti >= tj.
+ */
+#define RFP_TILE_BAND 8
+__device__ __forceinline__ void tri_tile_coords(int64_t t, int64_t T, int64_t &ti, int64_t &tj)
+{
+    const int64_t G = RFP_TILE_BAND;
+    int64_t b0 = 0;
+    for (;;) {
+        const int64_t g = (T - b0) < G ? (T - b0) : G;        /* rows in this band */
+        const int64_t cnt = b0 * g + g * (g + 1) / 2;
+        if (t < cnt || b0 + g >= T) {
+            if (t < b0 * g) {                                    /* rectangular part */
+                tj = t / g;
+                ti = b0 + t % g;
+            } else {                                             /* triangle part */
+                int64_t u = t - b0 * g, c = 0;
+                while (u >= g - c) { u -= g - c; ++c; }
+                tj = b0 + c;
+                ti = b0 + c + u;
+            }
+            return;
+        }
+        t -= cnt;
+        b0 += g;
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Handle                                                                    */
 /* ------------------------------------------------------------------------ */
