@@ -50,6 +50,8 @@ try:
         sadd_to_diagonal as _sadd_diag_C,
         sgen_residual as _sgen_residual_C,
         spftrs_refine as _spftrs_refine_C,
+        sgemm_strided as _sgemm_strided_C,
+        dgemm_strided as _dgemm_strided_C,
         dsfrk as _dsfrk_C,
         dpftrf as _dpftrf_C,
         dpftrs as _dpftrs_C,
@@ -446,6 +448,27 @@ def dstrttf(A, *, transr="T", uplo="U", n=None) -> torch.Tensor:
 def dstfttr(arf, *, transr="T", uplo="U", n=None) -> torch.Tensor:
     """FP64 stfttr (curfp/__init__.py:1378-1403)."""
     return _tfttr(arf, transr, uplo, n, True)
+
+
+def gemm_nt(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor = None, *, alpha: float = 1.0,
+            beta: float = 0.0) -> torch.Tensor:
+    """C = alpha * A @ B.T + beta * C on the library's MFMA GEMM (float32 or
+    float64; any strides with a unit stride in one direction)."""
+    if A.dtype != B.dtype or A.dtype not in (torch.float32, torch.float64):
+        raise ValueError("A and B must be both float32 or both float64")
+    if A.ndim != 2 or B.ndim != 2 or A.shape[1] != B.shape[1]:
+        raise ValueError(f"shape mismatch {tuple(A.shape)} x {tuple(B.shape)}^T")
+    M, K = A.shape
+    N = B.shape[0]
+    if C is None:
+        C = torch.empty(M, N, dtype=A.dtype, device=A.device)
+        beta = 0.0
+    if tuple(C.shape) != (M, N) or C.dtype != A.dtype:
+        raise ValueError("C has the wrong shape or dtype")
+    fn = _sgemm_strided_C if A.dtype == torch.float32 else _dgemm_strided_C
+    fn(_get_handle(), M, N, K, float(alpha), A.data_ptr(), A.stride(0), A.stride(1), B.data_ptr(), B.stride(0),
+       B.stride(1), float(beta), C.data_ptr(), C.stride(0), C.stride(1))
+    return C
 
 
 def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R=None) -> torch.Tensor:

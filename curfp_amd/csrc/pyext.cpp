@@ -147,6 +147,19 @@ PYBIND11_MODULE(_curfp_C, m)
         return py::make_tuple(iters, resid);
     });
 
+    m.def("sgemm_strided", [](Handle &h, int M, int N, int K, float alpha, int64_t A, int64_t a_rs, int64_t a_cs,
+                              int64_t B, int64_t b_rs, int64_t b_cs, float beta, int64_t C, int64_t c_rs,
+                              int64_t c_cs) {
+        check_status(curfpSgemmStrided(h.get(), M, N, K, &alpha, P<const float>(A), a_rs, a_cs, P<const float>(B),
+                                       b_rs, b_cs, &beta, P<float>(C), c_rs, c_cs));
+    });
+    m.def("dgemm_strided", [](Handle &h, int M, int N, int K, double alpha, int64_t A, int64_t a_rs, int64_t a_cs,
+                              int64_t B, int64_t b_rs, int64_t b_cs, double beta, int64_t C, int64_t c_rs,
+                              int64_t c_cs) {
+        check_status(curfpDgemmStrided(h.get(), M, N, K, &alpha, P<const double>(A), a_rs, a_cs, P<const double>(B),
+                                       b_rs, b_cs, &beta, P<double>(C), c_rs, c_cs));
+    });
+
     /* ---- FP64 ---- */
     m.def("dsfrk", [](Handle &h, int transr, int uplo, int trans, int n, int k, double alpha, int64_t A, int lda,
                       double beta, int64_t C) {

@@ -869,6 +869,40 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
 }
 
 /* ------------------------------------------------------------------------ */
+/* Strided GEMM                                                              */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+static curfpStatus_t gemm_strided_impl(curfpHandle_t h, int m, int n, int k, const T *alpha, const T *A,
+                                       int64_t a_rs, int64_t a_cs, const T *B, int64_t b_rs, int64_t b_cs,
+                                       const T *beta, T *C, int64_t c_rs, int64_t c_cs)
+{
+    RFP_CHECK_HANDLE(h);
+    if (m < 0 || n < 0 || k < 0 || !alpha || !beta) return CURFP_STATUS_INVALID_VALUE;
+    if (m == 0 || n == 0) return CURFP_STATUS_SUCCESS;
+    if (!C || (k > 0 && *alpha != T(0) && (!A || !B))) return CURFP_STATUS_INVALID_VALUE;
+    const bool unit_a = a_rs == 1 || a_cs == 1, unit_b = b_rs == 1 || b_cs == 1, unit_c = c_rs == 1 || c_cs == 1;
+    if (!unit_c || (k > 0 && (!unit_a || !unit_b))) return CURFP_STATUS_INVALID_VALUE;
+    GemmArgs g = gemm_args();
+    g.a = A; g.va = RfpView{0, a_rs, a_cs};
+    g.b = B; g.vb = RfpView{0, b_rs, b_cs};
+    g.c = C; g.vc = RfpView{0, c_rs, c_cs};
+    g.M = m; g.N = n; g.K = (*alpha == T(0)) ? 0 : k;
+    g.alpha = (double)*alpha; g.beta = (double)*beta;
+    return st_of(run_gemm<T>(h, g, Prec<T>::f64 ? RFP_MATH_F64 : RFP_MATH_F32));
+}
+
+extern "C" curfpStatus_t curfpSgemmStrided(curfpHandle_t h, int m, int n, int k, const float *alpha, const float *A,
+                                           int64_t a_rs, int64_t a_cs, const float *B, int64_t b_rs, int64_t b_cs,
+                                           const float *beta, float *C, int64_t c_rs, int64_t c_cs)
+{ return gemm_strided_impl<float>(h, m, n, k, alpha, A, a_rs, a_cs, B, b_rs, b_cs, beta, C, c_rs, c_cs); }
+
+extern "C" curfpStatus_t curfpDgemmStrided(curfpHandle_t h, int m, int n, int k, const double *alpha, const double *A,
+                                           int64_t a_rs, int64_t a_cs, const double *B, int64_t b_rs, int64_t b_cs,
+                                           const double *beta, double *C, int64_t c_rs, int64_t c_cs)
+{ return gemm_strided_impl<double>(h, m, n, k, alpha, A, a_rs, a_cs, B, b_rs, b_cs, beta, C, c_rs, c_cs); }
+
+/* ------------------------------------------------------------------------ */
 /* C ABI                                                                     */
 /* ------------------------------------------------------------------------ */
 

@@ -206,6 +206,22 @@ curfpStatus_t curfpSpftrsRefine(curfpHandle_t handle, curfpOperation_t transr,
                                 double *X, int ldx, double tol, int max_iter,
                                 int *iters, double *resid);
 
+/*
+ * curfpSgemmStrided / curfpDgemmStrided — the library's view GEMM on dense
+ * strided operands (no reference counterpart; used by the autograd backward):
+ *     C(i, j) = alpha * sum_k A(i, k) * B(j, k) + beta * C(i, j)
+ * with X(i, j) at X[i*x_rs + j*x_cs]; each of A, B, C needs a unit stride in
+ * one direction.  beta == 0 never reads C.
+ */
+curfpStatus_t curfpSgemmStrided(curfpHandle_t handle, int m, int n, int k,
+                                const float *alpha, const float *A, int64_t a_rs, int64_t a_cs,
+                                const float *B, int64_t b_rs, int64_t b_cs,
+                                const float *beta, float *C, int64_t c_rs, int64_t c_cs);
+curfpStatus_t curfpDgemmStrided(curfpHandle_t handle, int m, int n, int k,
+                                const double *alpha, const double *A, int64_t a_rs, int64_t a_cs,
+                                const double *B, int64_t b_rs, int64_t b_cs,
+                                const double *beta, double *C, int64_t c_rs, int64_t c_cs);
+
 /* ---------------------------- FP64 twins -------------------------------- */
 
 /* curfpDsfrk  — reference include/curfp.h:357-369, src/curfp_dssfrk.cpp:101-172. */
