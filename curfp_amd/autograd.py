@@ -26,19 +26,38 @@ import torch
 import curfp_amd as _c
 
 
-def _factor(A: torch.Tensor, lam: float, alpha: float, transr: str, uplo: str) -> torch.Tensor:
+def _factor(A: torch.Tensor, lam: float, alpha: float, transr: str, uplo: str, math: str) -> torch.Tensor:
     n = A.shape[0]
     f64 = A.dtype == torch.float64
     C = torch.empty(n * (n + 1) // 2, dtype=A.dtype, device=A.device)
-    (_c.dsfrk if f64 else _c.ssfrk)(A, C, alpha=alpha, transr=transr, uplo=uplo, trans="T")
+    if f64:
+        _c.dsfrk(A, C, alpha=alpha, transr=transr, uplo=uplo, trans="T")
+    elif math == "bf16":
+        # BF16 MFMA with FP32 accumulation; exact when A's entries are BF16 values
+        _c.ssfrk(A.to(torch.bfloat16), C, alpha=alpha, transr=transr, uplo=uplo, trans="T")
+    else:
+        with _c.math_mode(math):
+            _c.ssfrk(A, C, alpha=alpha, transr=transr, uplo=uplo, trans="T")
     _c.add_to_diagonal(C, lam, transr=transr, uplo=uplo)
-    (_c.dpftrf if f64 else _c.spftrf)(C, transr=transr, uplo=uplo, check=True)
+    if f64:
+        _c.dpftrf(C, transr=transr, uplo=uplo, check=True)
+    else:
+        with _c.math_mode(math):
+            _c.spftrf(C, transr=transr, uplo=uplo, check=True)
     return C
 
 
-def _solve(C: torch.Tensor, B: torch.Tensor, transr: str, uplo: str) -> torch.Tensor:
-    X = B.contiguous().clone()
-    (_c.dpftrs if C.dtype == torch.float64 else _c.spftrs)(C, X, transr=transr, uplo=uplo)
+def _solve(C: torch.Tensor, B: torch.Tensor, A: torch.Tensor, lam: float, alpha: float, transr: str, uplo: str,
+           refine_tol, max_iter: int) -> torch.Tensor:
+    if refine_tol is None:
+        X = B.to(C.dtype).contiguous().clone()
+        (_c.dpftrs if C.dtype == torch.float64 else _c.spftrs)(C, X, transr=transr, uplo=uplo)
+        return X
+    from .refine import refine_solve
+
+    X, info = refine_solve(C, A, lam, B, alpha=alpha, tol=refine_tol, max_iter=max_iter, transr=transr, uplo=uplo)
+    if not info["rel_residual"] <= refine_tol:
+        raise RuntimeError(f"iterative refinement did not reach {refine_tol:g}: {info}")
     return X
 
 
@@ -46,14 +65,16 @@ class KRRSolve(torch.autograd.Function):
     """Autograd function behind :func:`krr_solve`."""
 
     @staticmethod
-    def forward(ctx, A, lam, Y, alpha, transr, uplo, recompute):
+    def forward(ctx, A, lam, Y, alpha, transr, uplo, recompute, math, refine_tol, max_iter):
         A = A.contiguous()
         lam_f = float(lam)
-        C = _factor(A, lam_f, alpha, transr, uplo)
-        X = _solve(C, Y, transr, uplo)
+        C = _factor(A, lam_f, alpha, transr, uplo, math)
+        X = _solve(C, Y, A, lam_f, alpha, transr, uplo, refine_tol, max_iter)
         ctx.alpha, ctx.lam, ctx.transr, ctx.uplo = alpha, lam_f, transr, uplo
+        ctx.math, ctx.refine_tol, ctx.max_iter = math, refine_tol, max_iter
         ctx.lam_is_tensor = torch.is_tensor(lam)
         if recompute:
+            del C
             ctx.save_for_backward(A, X)
             ctx.has_factor = False
         else:
@@ -67,36 +88,48 @@ class KRRSolve(torch.autograd.Function):
             A, X, C = ctx.saved_tensors
         else:
             A, X = ctx.saved_tensors
-            C = _factor(A, ctx.lam, ctx.alpha, ctx.transr, ctx.uplo)
+            C = _factor(A, ctx.lam, ctx.alpha, ctx.transr, ctx.uplo, ctx.math)
         squeeze = X.ndim == 1
         X2 = X.unsqueeze(1) if squeeze else X
         Xb = Xbar.unsqueeze(1) if squeeze else Xbar
         # adjoint solve (C is symmetric): Ybar = C^{-1} Xbar
-        Ybar = _solve(C, Xb.to(X.dtype), ctx.transr, ctx.uplo)
+        Ybar = _solve(C, Xb, A, ctx.lam, ctx.alpha, ctx.transr, ctx.uplo, ctx.refine_tol, ctx.max_iter)
+        del C
         Abar = lam_bar = None
         if ctx.needs_input_grad[0]:
-            At = A.t()
+            Ag = A if A.dtype == X2.dtype else A.to(X2.dtype)
+            At = Ag.t()
             XtA = _c.gemm_nt(X2.t(), At)          # (nrhs, k) = X^T A
             YtA = _c.gemm_nt(Ybar.t(), At)        # (nrhs, k) = Ybar^T A
             Abar = _c.gemm_nt(Ybar, XtA.t(), alpha=-ctx.alpha)
-            Abar = _c.gemm_nt(X2, YtA.t(), Abar, alpha=-ctx.alpha, beta=1.0)
-        if ctx.needs_input_grad[1]:
+            Abar = _c.gemm_nt(X2, YtA.t(), Abar, alpha=-ctx.alpha, beta=1.0).to(A.dtype)
+        if ctx.needs_input_grad[1] and ctx.lam_is_tensor:
             lam_bar = -(Ybar * X2).sum()
-            if not ctx.lam_is_tensor:
-                lam_bar = None
         Ybar_out = Ybar[:, 0] if squeeze else Ybar
-        return Abar, lam_bar, Ybar_out, None, None, None, None
+        return Abar, lam_bar, Ybar_out, None, None, None, None, None, None, None
 
 
 def krr_solve(A: torch.Tensor, lam, Y: torch.Tensor, *, alpha: float = 1.0, transr: str = "T",
-              uplo: str = "U", recompute: bool = False) -> torch.Tensor:
+              uplo: str = "U", recompute: bool = False, math: str = "fp32", refine_tol: float = None,
+              max_iter: int = 30) -> torch.Tensor:
     """X solving (alpha*A@A.T + lam*I) X = Y, differentiable in A, lam and Y.
 
     A: (n, k) float32 or float64 device tensor (row-major, the ssfrk default
-    trans='T').  lam: float or 0-d tensor.  Y: (n, nrhs) or (n,), same dtype.
+    trans='T').  lam: float or 0-d tensor.  Y: (n, nrhs) or (n,).
+    math: arithmetic of ssfrk and the spftrf updates for a float32 A
+    ('fp32', 'bf16': A rounded to BF16, or 'bf16x3').  refine_tol: when set,
+    both the forward solve and the adjoint solve use FP64 iterative
+    refinement against the FP32 generator down to that relative residual, and
+    X (and the gradients w.r.t. Y) are float64.
+    recompute: release the factor after the forward and recompute it in the
+    backward.
     """
-    if A.dtype not in (torch.float32, torch.float64) or Y.dtype != A.dtype:
-        raise ValueError("A and Y must share a float32 or float64 dtype")
+    if A.dtype not in (torch.float32, torch.float64):
+        raise ValueError("A must be float32 or float64")
+    if A.dtype == torch.float64 and (math != "fp32" or refine_tol is not None):
+        raise ValueError("BF16 math and refinement apply to a float32 A")
     if Y.shape[0] != A.shape[0]:
         raise ValueError(f"Y has {Y.shape[0]} rows, A has {A.shape[0]}")
-    return KRRSolve.apply(A, lam, Y, float(alpha), transr, uplo, bool(recompute))
+    if refine_tol is None and Y.dtype != A.dtype:
+        raise ValueError("Y must have A's dtype")
+    return KRRSolve.apply(A, lam, Y, float(alpha), transr, uplo, bool(recompute), math, refine_tol, int(max_iter))
