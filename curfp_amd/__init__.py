@@ -115,6 +115,11 @@ def set_profiling(enable: bool) -> None:
     _get_handle().set_profiling(bool(enable))
 
 
+def workspace_bytes() -> int:
+    """Device memory held by the current device's handle (workspace + BF16 shadow)."""
+    return int(_get_handle().workspace_bytes())
+
+
 def kernel_stats(reset: bool = False) -> dict:
     """Per kernel class: launches, algorithmic flops / bytes, summed ms."""
     h = _get_handle()

@@ -43,6 +43,12 @@ public:
         check_status(curfpGetMathMode(h_, &m));
         return (int)m;
     }
+    size_t workspace_bytes() const
+    {
+        size_t b = 0;
+        check_status(curfpGetWorkspaceSize(h_, &b));
+        return b;
+    }
     void set_profiling(bool on) { check_status(curfpSetProfiling(h_, on ? 1 : 0)); }
     py::dict kernel_stats(int cls, bool reset)
     {
@@ -75,6 +81,7 @@ PYBIND11_MODULE(_curfp_C, m)
         .def("set_math_mode", &Handle::set_math_mode)
         .def("get_math_mode", &Handle::get_math_mode)
         .def("set_profiling", &Handle::set_profiling)
+        .def("workspace_bytes", &Handle::workspace_bytes)
         .def("kernel_stats", &Handle::kernel_stats, py::arg("cls"), py::arg("reset") = false)
         .def("__enter__", [](Handle &h) -> Handle & { return h; }, py::return_value_policy::reference)
         .def("__exit__", [](Handle &, py::object, py::object, py::object) {});

@@ -181,6 +181,14 @@ extern "C" curfpStatus_t curfpGetStream(curfpHandle_t handle, hipStream_t *strea
     return CURFP_STATUS_SUCCESS;
 }
 
+extern "C" curfpStatus_t curfpGetWorkspaceSize(curfpHandle_t handle, size_t *bytes)
+{
+    RFP_CHECK_HANDLE(handle);
+    if (!bytes) return CURFP_STATUS_INVALID_VALUE;
+    *bytes = handle->ws_bytes + handle->sh_bytes;
+    return CURFP_STATUS_SUCCESS;
+}
+
 extern "C" curfpStatus_t curfpSetMathMode(curfpHandle_t handle, curfpMathMode_t mode)
 {
     RFP_CHECK_HANDLE(handle);
@@ -843,6 +851,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
     if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, false, s)) != hipSuccess) return st_of(e);
     it = 1;
+    double best = INFINITY;
     for (;;) {
         /* R = B - C X (FP64), relative residual */
         e = gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, X, ldx, B, ldb, R, N,
@@ -857,6 +866,9 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
         const double nx = sqrt(hs[1]);
         rel = (normC > 0.0 && nx > 0.0) ? sqrt(hs[0]) / (normC * nx) : sqrt(hs[0]);
         if (!(rel > tol) || it >= max_iter || !std::isfinite(rel)) break;
+        /* refinement diverging (factor too inaccurate for this conditioning): stop */
+        if (it >= 4 && rel > 4.0 * best) break;
+        best = std::min(best, rel);
         /* D = F^{-1} fp32(R); X += D */
         if ((e = launch_cast_f64_f32(N, NR, R, N, D, N, s)) != hipSuccess) return st_of(e);
         if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);

@@ -73,6 +73,10 @@ curfpStatus_t curfpDestroy(curfpHandle_t handle);
 curfpStatus_t curfpSetStream(curfpHandle_t handle, hipStream_t stream);
 curfpStatus_t curfpGetStream(curfpHandle_t handle, hipStream_t *stream);
 
+/* Device memory currently held by the handle (cached workspace and the BF16
+ * shadow of the factor); no reference counterpart. */
+curfpStatus_t curfpGetWorkspaceSize(curfpHandle_t handle, size_t *bytes);
+
 /* Math mode for the S routines (no reference counterpart). */
 curfpStatus_t curfpSetMathMode(curfpHandle_t handle, curfpMathMode_t mode);
 curfpStatus_t curfpGetMathMode(curfpHandle_t handle, curfpMathMode_t *mode);
