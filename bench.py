@@ -149,6 +149,9 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
+    ap.add_argument("--copy-calib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (the known-size kernel that "
+                         "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -245,6 +248,13 @@ def main() -> int:
                     "launches_per_step": g["launches"] / args.steps,
                     "flops_per_launch": g["flops"] / g["launches"],
                     "avg_launch_ms": g["ms"] / g["launches"]}
+    if args.copy_calib > 0:
+        src = torch.ones(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if args.cpu_n > 0:
         log(f"cpu baseline n={args.cpu_n}")
