@@ -109,6 +109,64 @@ def run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, tol, max_iter, ev=None):
     return info, X, hist
 
 
+KERNEL_INFO = {
+    "gemm_f32": ("view_gemm_kernel<float> (FP32 MFMA v_mfma_f32_32x32x2f32)", "fp32"),
+    "gemm_bf16": ("gemm_bf16_rows_kernel / view_gemm_bf16_kernel (BF16 MFMA v_mfma_f32_32x32x16_bf16, "
+                  "FP32 accumulate)", "bf16"),
+    "gemm_f64": ("view_gemm_kernel<double> (FP64 MFMA v_mfma_f64_16x16x4f64)", "fp64"),
+    "potrf_tile": ("potrf_leaf_kernel (128-row Cholesky leaf + its inverse, FP32 MFMA)", "fp32"),
+    "trsv": ("block_solve_kernel (512-row block substitution, FP32 MFMA)", "fp32"),
+}
+
+
+def load_traffic() -> dict:
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (copy-calibrated), from the `TRAFFIC <class> <pipeline> <bytes>` lines of
+    the committed profile summaries."""
+    out = {}
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
+        return out
+    for name in sorted(os.listdir(pdir)):
+        if not name.endswith(".md"):
+            continue
+        with open(os.path.join(pdir, name)) as fh:
+            for ln in fh:
+                parts = ln.split()
+                if len(parts) == 4 and parts[0] == "TRAFFIC":
+                    out[(parts[1], parts[2])] = {"bytes": float(parts[3]), "source": f"profiles/{name}"}
+    return out
+
+
+def roofline_of(entry: dict, pipe: str, steps: int, traffic: dict, cls: str = None):
+    """Roofline object of one kernel class from the library's HIP-event kernel
+    statistics of the timed steps (launches, algorithmic flops and bytes, ms)."""
+    ks = entry.get("kernels") or {}
+    if cls is None:
+        cand = [c for c in ks if c in KERNEL_INFO and ks[c]["ms"] > 0]
+        if not cand:
+            return None
+        cls = max(cand, key=lambda c: ks[c]["ms"])
+    g = ks.get(cls)
+    if not g or g["ms"] <= 0 or not g["launches"]:
+        return None
+    desc, peak_key = KERNEL_INFO[cls]
+    achieved = g["flops"] / (g["ms"] * 1e-3)
+    t = traffic.get((cls, pipe))
+    return {"kernel": f"{desc}, {pipe} pipeline", "class": cls, "bound": "mfma",
+            "achieved": achieved / 1e12, "peak": PEAK[peak_key] / 1e12, "unit": "TFLOP/s",
+            "frac": achieved / PEAK[peak_key],
+            "traffic": t["bytes"] if t else None,
+            "traffic_source": t["source"] if t else None,
+            "algorithmic_bytes_per_launch": g["bytes"] / g["launches"],
+            "launches_per_step": g["launches"] / steps,
+            "flops_per_launch": g["flops"] / g["launches"],
+            "avg_launch_ms": g["ms"] / g["launches"],
+            "share_of_step": g["ms"] / steps / entry["profiled_ms_per_step"],
+            "measured": f"HIP events around every launch of this class over {steps} profiled step(s) "
+                        f"run after the timed steps"}
+
+
 def cpu_baseline(n, k, nrhs, seed=1):
     """The oracle chain (numpy + scipy BLAS/LAPACK on the RFP sub-blocks, FP32)
     on this host's cores, on a bounded sample n."""
@@ -149,6 +207,8 @@ def main() -> int:
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
+    ap.add_argument("--prof-steps", type=int, default=1,
+                    help="extra steps run with per-launch HIP-event kernel statistics (roofline source)")
     ap.add_argument("--copy-calib", type=int, default=0,
                     help="after the run, copy this many MiB device-to-device (the known-size kernel that "
                          "calibrates rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -192,11 +252,9 @@ def main() -> int:
             else:
                 run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, args.tol, args.max_iter)
         torch.cuda.synchronize()
-        curfp.set_profiling(True)
         per = []
         infos = []
         hist = None
-        torch.cuda.synchronize()
         t0 = time.perf_counter()
         for s in range(args.steps):
             ev = events()
@@ -211,8 +269,24 @@ def main() -> int:
             log(f"{pipe} step {s}: {sum(per[-1]):.1f} ms")
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / args.steps
-        stats = curfp.kernel_stats(reset=True)
-        curfp.set_profiling(False)
+        # kernel statistics: HIP events around every launch, over further steps of
+        # the same workload (the per-launch event brackets cost ~5-12 % of a step,
+        # so they stay out of the timed steps above)
+        stats, prof_ms = {}, None
+        if args.prof_steps > 0:
+            curfp.kernel_stats(reset=True)
+            curfp.set_profiling(True)
+            torch.cuda.synchronize()
+            tp = time.perf_counter()
+            for s in range(args.prof_steps):
+                if pipe == "fp32":
+                    run_fp32_step(curfp, A, C, B0, X, lam)
+                else:
+                    run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, args.tol, args.max_iter)
+            torch.cuda.synchronize()
+            prof_ms = (time.perf_counter() - tp) / args.prof_steps * 1e3
+            stats = curfp.kernel_stats(reset=True)
+            curfp.set_profiling(False)
         ms = sum(sum(p) for p in per) / len(per)
         res = rel_residual(curfp, A, lam, Xf.double(), B0.double(), normC)
         med = [sorted(p[i] for p in per)[len(per) // 2] for i in range(3)]
@@ -227,6 +301,8 @@ def main() -> int:
             "rel_residual": res,
             "info": infos[-1],
             "kernels": {kname: v for kname, v in stats.items() if v["launches"]},
+            "profiled_steps": args.prof_steps,
+            "profiled_ms_per_step": prof_ms,
         }
         if hist is not None:
             entry["refinement_residuals"] = hist
@@ -238,16 +314,11 @@ def main() -> int:
     best = min(ok, key=lambda p: result[p]["ms_per_step"]) if ok else pipelines[0]
     head = result[best]
 
-    g = result.get("fp32", head)["kernels"].get("gemm_f32") if "fp32" in result else None
-    roofline = None
-    if g and g["ms"] > 0:
-        achieved = g["flops"] / (g["ms"] * 1e-3)
-        roofline = {"kernel": "view_gemm_kernel<float> (FP32 MFMA v_mfma_f32_32x32x2f32), fp32 pipeline",
-                    "bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK["fp32"] / 1e12, "unit": "TFLOP/s",
-                    "frac": achieved / PEAK["fp32"], "traffic": None,
-                    "launches_per_step": g["launches"] / args.steps,
-                    "flops_per_launch": g["flops"] / g["launches"],
-                    "avg_launch_ms": g["ms"] / g["launches"]}
+    # roofline of the dominant kernel class (most time) of the headline pipeline,
+    # and of the plain path's FP32 GEMM (its MFMA utilisation against the FP32 peak)
+    traffic = load_traffic()
+    roofline = roofline_of(result.get(best, {}), best, args.prof_steps, traffic)
+    roofline_fp32 = roofline_of(result.get("fp32", {}), "fp32", args.prof_steps, traffic, cls="gemm_f32")
     if args.copy_calib > 0:
         src = torch.ones(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
@@ -278,6 +349,7 @@ def main() -> int:
                    "pipeline": best, "resid_target": args.tol},
         "seconds": head["ms_per_step"] * 1e-3,
         "roofline": roofline,
+        "roofline_fp32_path": roofline_fp32,
         "cpu_baseline": cpu,
         "pipelines": result,
     }

@@ -262,13 +262,32 @@ static double gemm_flops(const GemmArgs &g)
     return g.lower ? (double)g.M * (double)(g.M + 1) * (double)g.K : 2.0 * g.M * (double)g.N * (double)g.K;
 }
 
+/* Algorithmic HBM bytes of one view GEMM: each operand read once (A alone
+ * when B is the same view, as in a SYRK), C written once and read too when
+ * beta != 0, plus the BF16 shadow written by the epilogue.  Split-K partials
+ * are not algorithmic and are not counted. */
+template <typename T>
+static double gemm_bytes(const GemmArgs &g, RfpMath m)
+{
+    const bool bf16_in = g.a_bf16 || g.bf16_rows;
+    const bool x3 = g.bf16_rows && m == RFP_MATH_BF16X3;
+    const double ea = bf16_in ? 2.0 * (x3 ? 2 : 1) : (g.a_f32 ? 4.0 : (double)sizeof(T));
+    const double eb = bf16_in ? 2.0 * (x3 ? 2 : 1) : (g.b_f32 ? 4.0 : (double)sizeof(T));
+    const bool same = g.a == g.b && g.va.off == g.vb.off && g.va.rs == g.vb.rs && g.va.cs == g.vb.cs;
+    const double out = g.lower ? (double)g.M * (double)(g.M + 1) / 2.0 : (double)g.M * (double)g.N;
+    double bytes = (double)g.M * g.K * ea + (same ? 0.0 : (double)g.N * g.K * eb);
+    bytes += out * (double)sizeof(T) * (g.beta != 0.0 ? 2.0 : 1.0);
+    if (g.c_sh_hi) bytes += out * 2.0 * (g.c_sh_lo ? 2.0 : 1.0);
+    return bytes;
+}
+
 template <typename T>
 static hipError_t run_gemm(curfpHandle_t h, const GemmArgs &g, RfpMath m)
 {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     const int cls = Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64
                                  : ((m == RFP_MATH_F32 && !g.a_bf16) ? CURFP_KERNEL_GEMM_F32 : CURFP_KERNEL_GEMM_BF16);
-    return timed(h, cls, gemm_flops(g), 0.0, [&] { return Prec<T>::gemm(g, m, h->stream); });
+    return timed(h, cls, gemm_flops(g), gemm_bytes<T>(g, m), [&] { return Prec<T>::gemm(g, m, h->stream); });
 }
 
 static RfpMath math_of(curfpHandle_t h)
