@@ -358,52 +358,105 @@ __device__ __forceinline__ float bcast(float v, int lane)
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
+/* order LDS traffic between the lanes of one wave */
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 /*
  * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, one row
- * per lane (lanes 32..63 idle), write L into D's lower triangle and inv(L)
- * into Xd.  Returns the failing local column or -1 (wave-uniform).
+ * per lane (lanes 32..63 duplicate lanes 0..31 and store nothing), write L
+ * into D's lower triangle and inv(L) into Xd's lower triangle.  W is a
+ * 16-byte aligned LDS scratch of BS floats.  Returns the failing local
+ * column or -1 (wave-uniform).
+ *
+ * Right-looking: at column j every lane publishes its entry of the column
+ * in LDS and reads the whole column back as 16-byte broadcasts
+ * c[k] = A(k, j), so lane i applies  r[k] -= (L(i,j) / L(j,j)) * c[k]  to its
+ * row with no cross-lane register traffic.  The update is applied to all
+ * entries of a row without a per-entry predicate: entries right of the
+ * diagonal (k > i) absorb updates but are never read (step j reads c[k]
+ * only for k >= j, i.e. rows at or below the diagonal), so the block's upper
+ * triangle in LDS holds garbage afterwards, which no later step reads.
+ * 1/sqrt(pivot) is v_rsq_f32 (1 ulp); the pivot's own lane computes
+ * L(j,j) = pivot * rsq(pivot).
+ * The inverse rides along: lane c also holds column c of X = inv(L),
+ * starting from e_c, and applies the same column-j step to it
+ * (x[j] /= L(j,j), then x[k] -= L(k,j) x[j] for k > j), so its entries above
+ * the diagonal come out as exact zeros.
  */
-__device__ int factor_diag(float *D, float *Xd, int lane)
+__device__ int factor_diag(float *D, float *Xd, float *W, int lane)
 {
     const int i = lane & 31;
-    const bool act = lane < 32;
-    float r[BS];
+    const bool own = lane < 32;
+    float r[BS], x[BS];
+    const float fi = (float)i;
 #pragma unroll
-    for (int k = 0; k < BS; ++k) r[k] = (act && k <= i) ? D[i * LP + k] : 0.f;
+    for (int k = 0; k < BS; ++k) {
+        r[k] = D[i * LP + k];
+        x[k] = fmaxf(0.f, 1.f - fabsf(fi - (float)k));   /* delta(k, i) */
+    }
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
-        const float piv = bcast(r[j], j);
+        if (own) W[i] = r[j];
+        wave_sync();
+        float c[BS];
+#pragma unroll
+        for (int g = j / 4; g < BS / 4; ++g) {
+            const float4 t = reinterpret_cast<const float4 *>(W)[g];
+            c[4 * g] = t.x; c[4 * g + 1] = t.y; c[4 * g + 2] = t.z; c[4 * g + 3] = t.w;
+        }
+        wave_sync();
+        const float piv = c[j];
         if (!(piv > 0.f)) return j;
-        const float ljj = sqrtf(piv);
-        const float rinv = 1.f / ljj;
-        r[j] = (i == j) ? ljj : ((i > j) ? r[j] * rinv : r[j]);
+        const float rinv = __builtin_amdgcn_rsqf(piv);
+        const float lij = r[j] * rinv;
+        const float t = lij * rinv;
+        r[j] = lij;
+        x[j] *= rinv;
+        const float u = x[j] * rinv;
 #pragma unroll
         for (int k = j + 1; k < BS; ++k) {
-            const float lkj = bcast(r[j], k);
-            if (i >= k) r[k] -= r[j] * lkj;
+            r[k] = fmaf(-t, c[k], r[k]);
+            x[k] = fmaf(-u, c[k], x[k]);
         }
     }
-    if (act) {
+    if (own) {
 #pragma unroll
-        for (int k = 0; k < BS; ++k)
-            if (k <= i) D[i * LP + k] = r[k];
-    }
-    /* inverse, one column per lane: x(c) = e_c solved by forward substitution;
-     * lane c holds column c of X in registers, L entries broadcast from r */
-    float x[BS];
-#pragma unroll
-    for (int q = 0; q < BS; ++q) {
-        /* row q of X:  x[q] = (delta(q, c) - sum_{k<q} L(q, k) x[k]) / L(q, q) */
-        float acc = (q == i) ? 1.f : 0.f;
-#pragma unroll
-        for (int k = 0; k < q; ++k) acc -= bcast(r[k], q) * x[k];
-        x[q] = acc / bcast(r[q], q);
-    }
-    if (act) {
-#pragma unroll
-        for (int q = 0; q < BS; ++q) Xd[q * LP + i] = (q >= i) ? x[q] : 0.f;
+        for (int k = 0; k < BS; ++k) {
+            D[i * LP + k] = r[k];
+            Xd[k * LP + i] = x[k];
+        }
     }
     return -1;
+}
+
+/* L(i, j) -= L(i, p) L(j, p)^T for one 32 x 32 block (lower part only when i == j) */
+__device__ __forceinline__ void trail_block(float *L, int i, int j, int p, int lane)
+{
+    f32x16_t acc;
+    zero(acc);
+    mma(acc, L + (i * BS) * LP + p * BS, LP, 1, L + (j * BS) * LP + p * BS, 1, LP, lane);
+    store(acc, L + (i * BS) * LP + j * BS, LP, -1.f, true, lane, i == j);
+}
+
+/* X(i, j) = -X(i, i) * sum_{k=j}^{i-1} L(i, k) X(k, j) for one block below
+ * the diagonal of the inverse; Tw is the wave's own LDS scratch (pitch 33) */
+__device__ __forceinline__ void inv_block(const float *L, float *X, float *Tw, int i, int j, int lane)
+{
+    f32x16_t acc;
+    zero(acc);
+    for (int k = j; k < i; ++k)
+        mma(acc, L + (i * BS) * LP + k * BS, LP, 1, X + (k * BS) * LP + j * BS, LP, 1, lane);
+    store(acc, Tw, 33, 1.f, false, lane);
+    wave_sync();
+    f32x16_t acc2;
+    zero(acc2);
+    mma(acc2, X + (i * BS) * LP + i * BS, LP, 1, Tw, 33, 1, lane);
+    store(acc2, X + (i * BS) * LP + j * BS, LP, -1.f, false, lane);
+    wave_sync();
 }
 
 __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v, int m, int *info, int info_base,
@@ -411,49 +464,85 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
 {
     __shared__ float L[NB * LP];
     __shared__ float X[NB * LP];
-    __shared__ float T[4][BS * 33];
+    __shared__ __attribute__((aligned(16))) float T[4][BS * 33];
     __shared__ int fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (*info != INT_MAX) return;
-
-    /* load the lower triangle (all loads in flight before the LDS stores);
-     * identity padding beyond m */
+    /* Load the tile (all loads in flight before the LDS stores, the check
+     * of info among them).  Only the lower triangle is read by the
+     * factorisation; an entry above the diagonal is loaded from the diagonal
+     * of its row instead (always inside the view) and its value is never
+     * used.  Rows / columns at or beyond m are clamped the same way and then
+     * replaced by identity padding. */
     {
         constexpr int PER = NB * NB / 256;
+        const int mc = m - 1;
         const bool colv = v.rs == 1;
         float x[PER];
+        if (colv) {
+            /* column view: lanes run down a column */
+            const int i = tid % NB, j0 = tid / NB, ic = min(i, mc);
+            const float *p = base + v.off + ic;
 #pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int idx = tid + 256 * e;
-            const int i = colv ? idx % NB : idx / NB, j = colv ? idx / NB : idx % NB;
-            float t = 0.f;
-            if (i < m && j < m) {
-                if (i >= j) t = base[v.off + (int64_t)i * v.rs + (int64_t)j * v.cs];
-            } else if (i == j) {
-                t = 1.f;
+            for (int e = 0; e < PER; ++e) x[e] = p[(int64_t)min(j0 + 2 * e, ic) * v.cs];
+        } else {
+            /* row view: lanes run along a row */
+            const int j = tid % NB, i0 = tid / NB;
+            const float *p = base + v.off;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) {
+                const int ic = min(i0 + 2 * e, mc);
+                x[e] = p[(int64_t)ic * v.rs + min(j, ic) * v.cs];
             }
-            x[e] = t;
         }
+        if (*info != INT_MAX) return;
+        if (colv) {
+            const int i = tid % NB, j0 = tid / NB;
 #pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int idx = tid + 256 * e;
-            const int i = colv ? idx % NB : idx / NB, j = colv ? idx / NB : idx % NB;
-            L[i * LP + j] = x[e];
-            X[i * LP + j] = 0.f;
+            for (int e = 0; e < PER; ++e) L[i * LP + j0 + 2 * e] = x[e];
+        } else {
+            const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) L[(i0 + 2 * e) * LP + j] = x[e];
+        }
+        if (m < NB) {
+            __syncthreads();
+            for (int idx = tid; idx < NB * NB; idx += 256) {
+                const int i = idx / NB, j = idx % NB;
+                if (i >= m || j >= m) L[i * LP + j] = (i == j) ? 1.f : 0.f;
+            }
         }
     }
     if (tid == 0) fail = -1;
     __syncthreads();
 
+    /*
+     * Blocked right-looking over the 4 x 4 grid of 32 x 32 blocks.  Step p:
+     *   A: wave 0 applies step p-1's update to the diagonal block (p, p) and
+     *      factors it (with its inverse); meanwhile waves 1..3 apply step
+     *      p-1's other trailing updates and assemble row p-1 of the inverse
+     *      (its blocks need only earlier rows and X(p-1, p-1));
+     *   B: panel  L(i, p) := L(i, p) X(p, p)^T  for the blocks below.
+     */
     for (int p = 0; p < NBLK; ++p) {
-        float *Dpp = L + (p * BS) * LP + p * BS;
         if (wave == 0) {
-            const int f = factor_diag(Dpp, X + (p * BS) * LP + p * BS, lane);
+            if (p > 0) {
+                trail_block(L, p, p, p - 1, lane);
+                wave_sync();
+            }
+            const int f = factor_diag(L + (p * BS) * LP + p * BS, X + (p * BS) * LP + p * BS, T[0], lane);
             if (lane == 0 && f >= 0) fail = p * BS + f;
+        } else if (p > 0) {
+            int blk = 0;
+            for (int i = p; i < NBLK; ++i)
+                for (int j = p; j <= i; ++j) {
+                    if (i == p && j == p) continue;
+                    if (blk++ % 3 == wave - 1) trail_block(L, i, j, p - 1, lane);
+                }
+            if (inv)
+                for (int j = wave - 1; j < p - 1; j += 3) inv_block(L, X, T[wave], p - 1, j, lane);
         }
         __syncthreads();
         if (fail >= 0) break;
-        /* panel: L(i, p) := L(i, p) * X(p, p)^T for i > p, one wave per block */
         const float *Xpp = X + (p * BS) * LP + p * BS;
         for (int i = p + 1 + wave; i < NBLK; i += 4) {
             float *Lip = L + (i * BS) * LP + p * BS;
@@ -463,30 +552,23 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
             store(acc, Lip, LP, 1.f, false, lane);      /* the wave read all of Lip before writing */
         }
         __syncthreads();
-        /* trailing: L(i, j) -= L(i, p) L(j, p)^T for p < j <= i */
-        int blk = 0;
-        for (int i = p + 1; i < NBLK; ++i)
-            for (int j = p + 1; j <= i; ++j, ++blk) {
-                if ((blk & 3) != wave) continue;
-                f32x16_t acc;
-                zero(acc);
-                mma(acc, L + (i * BS) * LP + p * BS, LP, 1, L + (j * BS) * LP + p * BS, 1, LP, lane);
-                store(acc, L + (i * BS) * LP + j * BS, LP, -1.f, true, lane, i == j);
-            }
-        __syncthreads();
     }
 
-    /* write L back (also after a failure: LAPACK leaves a partial factor) */
+    /* write L back (also after a failure: LAPACK leaves a partial factor);
+     * the stores drain while the inverse is assembled */
     if (v.rs == 1) {
-        for (int idx = tid; idx < NB * NB; idx += 256) {
-            const int i = idx % NB, j = idx / NB;
-            if (i < m && j < m && i >= j) base[v.off + i + (int64_t)j * v.cs] = L[i * LP + j];
+        const int i = tid % NB, j0 = tid / NB;
+        if (i < m) {
+            float *p = base + v.off + i + (int64_t)j0 * v.cs;
+            const int64_t step = 2 * v.cs;
+            for (int j = j0; j <= i; j += 2, p += step) *p = L[i * LP + j];
         }
     } else {
-        for (int idx = tid; idx < NB * NB; idx += 256) {
-            const int j = idx % NB, i = idx / NB;
-            if (i < m && j < m && i >= j) base[v.off + (int64_t)i * v.rs + j] = L[i * LP + j];
-        }
+        const int j = tid % NB, i0 = tid / NB;
+        float *p = base + v.off + (int64_t)i0 * v.rs + j;
+        const int64_t step = 2 * v.rs;
+        for (int i = i0; i < m; i += 2, p += step)
+            if (i >= j) *p = L[i * LP + j];
     }
     if (fail >= 0) {
         if (tid == 0) atomicMin(info, info_base + fail + 1);
@@ -494,28 +576,18 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
     }
     if (!inv) return;
 
-    /* blocks of the inverse below the diagonal, by distance d = i - j */
-    for (int d = 1; d < NBLK; ++d) {
-        for (int j = wave; j + d < NBLK; j += 4) {
-            const int i = j + d;
-            f32x16_t acc;
-            zero(acc);
-            for (int k = j; k < i; ++k)
-                mma(acc, L + (i * BS) * LP + k * BS, LP, 1, X + (k * BS) * LP + j * BS, LP, 1, lane);
-            store(acc, T[wave], 33, 1.f, false, lane);
-            __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            /* X(i, j) = -X(i, i) T */
-            f32x16_t acc2;
-            zero(acc2);
-            mma(acc2, X + (i * BS) * LP + i * BS, LP, 1, T[wave], 33, 1, lane);
-            store(acc2, X + (i * BS) * LP + j * BS, LP, -1.f, false, lane);
-        }
-        __syncthreads();
-    }
-    for (int idx = tid; idx < NB * NB; idx += 256) {
+    /* the last row of the inverse */
+    if (wave < NBLK - 1) inv_block(L, X, T[wave], NBLK - 1, wave, lane);
+    __syncthreads();
+    /* dense row-major inverse, zero above the diagonal and beyond m */
+#pragma unroll 1
+    for (int e = 0; e < NB * NB / 1024; ++e) {
+        const int idx = 4 * (tid + 256 * e);
         const int i = idx / NB, j = idx % NB;
-        inv[idx] = (i < m && j < m && j <= i) ? X[i * LP + j] : 0.f;
+        float w[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[u] = (i < m && j + u <= i) ? X[i * LP + j + u] : 0.f;
+        *reinterpret_cast<float4 *>(inv + idx) = make_float4(w[0], w[1], w[2], w[3]);
     }
 }
 
