@@ -178,11 +178,10 @@ struct SliceLoader {
     }
 };
 
-template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M, bool AKC, bool BKC>
+template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M, int BK, bool AKC, bool BKC>
 __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
 {
     using MF = Mfma<T>;
-    constexpr int BK = 16;
     constexpr int WAVES_N = 4 / WAVES_M;
     constexpr int WTM = BM / WAVES_M, WTN = BN / WAVES_N;
     constexpr int FM = WTM / MF::TM, FN = WTN / MF::TN;
@@ -308,7 +307,7 @@ __global__ void splitk_reduce_kernel(const T *P, int64_t M, int64_t N, int64_t s
 /* Host launchers                                                            */
 /* ------------------------------------------------------------------------ */
 
-template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M>
+template <typename T, typename AS, typename BS, int BM, int BN, int WAVES_M, int BK = 16>
 static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
 {
     KParams p;
@@ -345,7 +344,7 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
          * kernel sums them into C with alpha / beta */
         splits = g.splits;
         p.kchunk = (g.K + splits - 1) / splits;
-        p.kchunk = (p.kchunk + 15) / 16 * 16;
+        p.kchunk = (p.kchunk + BK - 1) / BK * BK;
         splits = (g.K + p.kchunk - 1) / p.kchunk;
         if (splits < 1) splits = 1;
         p.c = g.partial; p.c_off = 0; p.c_rs = 1; p.c_cs = g.M;
@@ -354,10 +353,10 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
         p.sh_hi = p.sh_lo = nullptr;
     }
     dim3 grid((unsigned)p.ntiles, (unsigned)splits), block(256);
-    if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, true, true>), grid, block, 0, s, p);
-    else if (akc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, true, false>), grid, block, 0, s, p);
-    else if (bkc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, false, true>), grid, block, 0, s, p);
-    else             hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, false, false>), grid, block, 0, s, p);
+    if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, true, true>), grid, block, 0, s, p);
+    else if (akc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, true, false>), grid, block, 0, s, p);
+    else if (bkc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, false, true>), grid, block, 0, s, p);
+    else             hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, false, false>), grid, block, 0, s, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess || splits == 1 || p.c != g.partial) return e;
     const int64_t total = g.M * g.N;
@@ -370,13 +369,34 @@ int gemm_inplace_limit(bool f64) { (void)f64; return 128; }
 
 hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s);
 
+/* Number of 128 x 128 output tiles of a GEMM (on or below the diagonal for a
+ * lower-triangular output). */
+static int64_t tiles128(const GemmArgs &g)
+{
+    const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
+    return g.lower ? tm * (tm + 1) / 2 : tm * tn;
+}
+
+/*
+ * Tile choice.  A workgroup's time on a small problem is its own latency:
+ * its share of MFMA work plus one exposed global-load round trip per k-slice.
+ * Problems that give fewer than SMALL_TILES 128 x 128 tiles (most of the
+ * launches deep in the factorisation's recursion) therefore use 64 x 64
+ * tiles with BK = 32 — four times the workgroups, a quarter of the work
+ * each, half the k-slices — and the in-place TRSM-by-inverse (C aliases A,
+ * N <= 128) uses 32-row tiles spanning all N.
+ */
+constexpr int64_t SMALL_TILES = 128;
+
 hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     if (math != RFP_MATH_F32 || g.a_bf16) return launch_gemm_bf16(g, math, s);
+    if (g.inplace == 1) return launch_cfg<float, float, float, 32, 128, 1, 32>(g, s);
+    const bool small = g.inplace != 2 && tiles128(g) < SMALL_TILES;
+    if (small) return launch_cfg<float, float, float, 64, 64, 2, 32>(g, s);
     /* narrow outputs (the solves' nrhs columns) use a 128 x 64 tile */
-    if (!g.lower && g.N <= 64 && g.inplace != 1)
-        return launch_cfg<float, float, float, 128, 64, 4>(g, s);
+    if (!g.lower && g.N <= 64) return launch_cfg<float, float, float, 128, 64, 4>(g, s);
     return launch_cfg<float, float, float, 128, 128, 2>(g, s);
 }
 
@@ -389,7 +409,9 @@ hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s)
         if (g.b_f32) return launch_cfg<double, float, float, 128, 64, 4>(g, s);
         return launch_cfg<double, float, double, 128, 64, 4>(g, s);
     }
-    if (!g.lower && g.N <= 64 && g.inplace != 1)
-        return launch_cfg<double, double, double, 128, 64, 4>(g, s);
+    if (g.inplace == 1) return launch_cfg<double, double, double, 32, 128, 1, 32>(g, s);
+    const bool small = g.inplace != 2 && tiles128(g) < SMALL_TILES;
+    if (small) return launch_cfg<double, double, double, 64, 64, 2, 32>(g, s);
+    if (!g.lower && g.N <= 64) return launch_cfg<double, double, double, 128, 64, 4>(g, s);
     return launch_cfg<double, double, double, 128, 128, 2>(g, s);
 }

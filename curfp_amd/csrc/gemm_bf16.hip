@@ -268,7 +268,8 @@ hipError_t launch_typed(const BKParams &p, bool akc, bool bkc, hipStream_t s)
  * rfp_drivers.hip): A(i, k) = a[a_off + i*a_ld + k], rows 16-byte aligned.
  * Tile BT x BT (BT = 256: 4 waves of 128 x 128, 4 x 4 MFMA blocks each, 256
  * accumulator registers per lane; BT = 128 for split-BF16, whose hi + lo
- * operands double the LDS), BK = 32, double-buffered LDS, one 16-byte global
+ * operands double the LDS; BT = 64 for small problems), BK = 32 or 64,
+ * double-buffered LDS, one 16-byte global
  * load and one 16-byte LDS store per vector, one 16-byte LDS read per MFMA
  * operand.
  */
@@ -473,9 +474,16 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
         if (g.lower && g.M != g.N) return hipErrorInvalidValue;
         const bool x3 = math == RFP_MATH_BF16X3;
         if (x3 && (!g.a_lo || !g.b_lo)) return hipErrorInvalidValue;
+        /* 256 x 256 tiles for the large updates; 64 x 64 tiles (BK = 64) for
+         * problems of fewer than 128 tiles of 128 x 128, where a workgroup's
+         * own latency sets the time (four times the workgroups, a quarter of
+         * the work and half the k-slices each) */
+        const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
+        const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) < 128;
         const bool big = g.M >= 2048 && g.N >= 2048;
-        if (x3) return launch_rows<128, true, 32>(g, s);
-        return big ? launch_rows<256, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
+        if (x3) return small ? launch_rows<64, true, 64>(g, s) : launch_rows<128, true, 32>(g, s);
+        if (big) return launch_rows<256, false, 64>(g, s);
+        return small ? launch_rows<64, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
     }
     BKParams p;
     p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;
