@@ -1,0 +1,302 @@
+/*
+ * gemm_bf16_dma.hip — the large BF16 updates of ssfrk (BF16 input) and of
+ * spftrf's SYRK / GEMM / TRSM updates in BF16 mode:
+ *
+ *   C(i, j) = alpha * sum_k A(i, k) * B(j, k) + beta * C(i, j)
+ *
+ * A and B are BF16 row blocks with k contiguous and 16-byte aligned rows
+ * (the factor's BF16 shadow, or the caller's BF16 matrix); C is an FP32 view
+ * into the RFP array.
+ *
+ * Structure: 256 x 256 output tile, 4 wave64s (2 x 2) of 128 x 128 each —
+ * 4 x 4 blocks of v_mfma_f32_32x32x16_bf16, 256 FP32 accumulators per lane.
+ * k-slices of BK = 64 travel global -> LDS by buffer_load_dwordx4 ... lds
+ * (no VGPR staging, no LDS store instructions): one wave instruction fills
+ * 1 KB of LDS (8 rows of 128 bytes) with lane l's 16 bytes at offset 16 l,
+ * and each lane chooses which global 16-byte chunk it fetches so that LDS
+ * chunk p of row r holds global chunk p ^ ((r >> 1) & 7).  A 128-byte row
+ * spans half of the 64 LDS banks, so with that XOR swizzle the 16 lanes of
+ * one LDS cycle (16 consecutive rows, one chunk each) hit all 64 banks once.  Per-tile buffer descriptors bound the rows (num_records covers
+ * exactly the valid rows) and chunks at or beyond K get an offset outside
+ * the descriptor, so the edges of the problem read as zeros without
+ * branches.  Two LDS buffers per operand (128 KB in all) alternate with the
+ * loop unrolled by two, so each buffer is a distinct LDS object and the next
+ * slice's loads stay in flight during the current slice's MFMAs; operand
+ * fragments are double-buffered in registers across the four 16-wide k
+ * steps of a slice.
+ */
+#include "rfp_internal.h"
+
+namespace {
+
+typedef float  f32x16 __attribute__((ext_vector_type(16)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+#define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
+
+constexpr int BT = 256;              /* tile rows / columns */
+constexpr int BK = 32;               /* k per slice */
+constexpr int ROWB = BK * 2;         /* bytes of one row of a slice: 4 chunks of 16 */
+constexpr int SLICE = BT * ROWB;     /* bytes of one operand slice: 16 KB */
+constexpr int NBUF = 4;              /* LDS buffers: slices s .. s + 3 */
+constexpr int RSRC_FLAGS = 0x00020000;
+
+struct DmaParams {
+    const unsigned short *a; int64_t a_ld;   /* element (i, k) at a[i*a_ld + k] */
+    const unsigned short *b; int64_t b_ld;
+    float *c; int64_t c_off, c_rs, c_cs;
+    int64_t M, N, K;
+    float alpha, beta;
+    int64_t tiles_m, tiles_n, ntiles;
+    int lower;
+};
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t pid, int64_t total)
+{
+    const int64_t X = 8;
+    const int64_t q = total / X, r = total % X;
+    const int64_t x = pid % X, w = pid / X;
+    return x * q + (x < r ? x : r) + w;
+}
+
+__device__ __forceinline__ void tile_coords(const DmaParams &p, int64_t t, int64_t &tm, int64_t &tn)
+{
+    if (p.lower) {
+        tri_tile_coords(t, p.tiles_m, tm, tn);
+    } else {
+        const int64_t GROUP = 8;
+        const int64_t in_group = GROUP * p.tiles_n;
+        const int64_t gid = t / in_group;
+        const int64_t first = gid * GROUP;
+        const int64_t gsz = (p.tiles_m - first) < GROUP ? (p.tiles_m - first) : GROUP;
+        const int64_t loc = t % in_group;
+        tm = first + loc % gsz;
+        tn = loc / gsz;
+    }
+}
+
+/* Loads of one operand's k-slice [k0, k0 + BK): 16 groups of 16 rows, 4 per
+ * wave (q = 0..3).  Lane l of group g handles row 16 g + l / 4 and LDS chunk
+ * l % 4, which holds global chunk (l % 4) ^ ((row >> 2) & 3) = (l % 4) ^ (l / 16). */
+struct SliceDma {
+    int rowoff;   /* (l / 4) * ld2: byte offset of the lane's row within a group */
+    int c8;       /* 8 * global chunk of the lane */
+
+    __device__ __forceinline__ void init(int ld2, int lane)
+    {
+        c8 = 8 * ((lane & 3) ^ (lane >> 4));
+        rowoff = (lane >> 2) * ld2;
+    }
+    __device__ __forceinline__ void issue(unsigned char *lds, __amdgpu_buffer_rsrc_t rs, int ld2, int k0, int K,
+                                          int q, int wave) const
+    {
+        const int g = q * 4 + wave;
+        const int kk = k0 + c8;
+        /* K % 8 == 0: a chunk is wholly inside or outside [0, K) */
+        const int off = kk < K ? 16 * g * ld2 + rowoff + kk * 2 : (int)0x80000000;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, LDS_PTR(lds + g * 1024), 16, off, 0, 0, 0);
+    }
+};
+
+/* Operand fragments of one 16-wide k step t of a slice: lane l reads row
+ * l % 32 of each 32-row block, global chunk 2 t + l / 32 (swizzled). */
+struct Frags {
+    bf16x8 a[4], b[4];
+};
+
+__device__ __forceinline__ int frag_off(int t, int lane)
+{
+    /* row = 32 i + (lane & 31) (+ 128 wm): (row >> 2) & 3 == ((lane & 31) >> 2) & 3 */
+    return (lane & 31) * ROWB + ((((2 * t) + (lane >> 5)) ^ (((lane & 31) >> 2) & 3)) << 4);
+}
+
+/*
+ * acc += the wave's 128 x 128 block of the slice in (As, Bs), two 16-wide k
+ * steps.  All 16 operand reads of the slice are issued first; the loads of
+ * slice s + 3 (into Na, Nb) are then issued between the groups of four
+ * MFMAs of step 0 (the order is fixed with scheduling barriers).  No LDS
+ * read follows a global -> LDS load within a slice, so the compiler has no
+ * reason to wait on the loads in flight.
+ */
+__device__ __forceinline__ void mma_slice(f32x16 (&acc)[4][4], const unsigned char *As, const unsigned char *Bs,
+                                          unsigned char *Na, unsigned char *Nb, const SliceDma &da,
+                                          const SliceDma &db, __amdgpu_buffer_rsrc_t rsa,
+                                          __amdgpu_buffer_rsrc_t rsb, int lda2, int ldb2, int k_next, int K,
+                                          int wm, int wn, int wave, int lane)
+{
+    const unsigned char *arow = As + wm * 128 * ROWB;
+    const unsigned char *brow = Bs + wn * 128 * ROWB;
+    Frags f[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int o = frag_off(t, lane);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            f[t].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
+            f[t].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
+        }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (t == 0) {
+                da.issue(Na, rsa, lda2, k_next, K, i, wave);
+                db.issue(Nb, rsb, ldb2, k_next, K, i, wave);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[t].a[i], f[t].b[j], acc[i][j], 0, 0, 0);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+}
+
+/* wait until at most 16 of this wave's global -> LDS loads (two slices: 4 A +
+ * 4 B instructions each) are outstanding, i.e. the oldest slice has landed;
+ * lgkm and exp counters are not waited on */
+__device__ __forceinline__ void wait_slice()
+{
+    __builtin_amdgcn_s_waitcnt((1 << 14) | (0xF << 8) | (0x7 << 4) | 0x0);
+}
+
+__global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
+{
+    /* one LDS object per buffer (A slice, then B slice) */
+    __shared__ __attribute__((aligned(1024))) unsigned char s0[2 * SLICE];
+    __shared__ __attribute__((aligned(1024))) unsigned char s1[2 * SLICE];
+    __shared__ __attribute__((aligned(1024))) unsigned char s2[2 * SLICE];
+    __shared__ __attribute__((aligned(1024))) unsigned char s3[2 * SLICE];
+    unsigned char *const sA0 = s0, *const sB0 = s0 + SLICE;
+    unsigned char *const sA1 = s1, *const sB1 = s1 + SLICE;
+    unsigned char *const sA2 = s2, *const sB2 = s2 + SLICE;
+    unsigned char *const sA3 = s3, *const sB3 = s3 + SLICE;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;
+
+    int64_t tm, tn;
+    tile_coords(p, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
+    /* the tile is workgroup-uniform: keep it (and the buffer descriptors
+     * built from it) in scalar registers */
+    const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * BT;
+    const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * BT;
+    const int64_t ra = (p.M - m0) < BT ? (p.M - m0) : BT;
+    const int64_t rb = (p.N - n0) < BT ? (p.N - n0) : BT;
+    const int lda2 = (int)(p.a_ld * 2), ldb2 = (int)(p.b_ld * 2);
+    const __amdgpu_buffer_rsrc_t rsa =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(p.a + m0 * p.a_ld), 0, (int)(ra * lda2), RSRC_FLAGS);
+    const __amdgpu_buffer_rsrc_t rsb =
+        __builtin_amdgcn_make_buffer_rsrc((void *)(p.b + n0 * p.b_ld), 0, (int)(rb * ldb2), RSRC_FLAGS);
+    const int K = (int)p.K;
+
+    f32x16 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    SliceDma da, db;
+    da.init(lda2, lane);
+    db.init(ldb2, lane);
+    const int nk = (K + BK - 1) / BK;
+    if (nk > 0) {
+        /* slices 0, 1, 2 into buffers 0, 1, 2 (a slice at or beyond K loads
+         * as zeros) */
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            da.issue(sA0, rsa, lda2, 0, K, q, wave);
+            db.issue(sB0, rsb, ldb2, 0, K, q, wave);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            da.issue(sA1, rsa, lda2, BK, K, q, wave);
+            db.issue(sB1, rsb, ldb2, BK, K, q, wave);
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            da.issue(sA2, rsa, lda2, 2 * BK, K, q, wave);
+            db.issue(sB2, rsb, ldb2, 2 * BK, K, q, wave);
+        }
+    }
+    /* Slice s sits in buffer s % 4 and is multiplied while slices s + 1 and
+     * s + 2 are in flight; its step 0 issues the loads of slice s + 3 into
+     * buffer (s + 3) % 4, the one slice s - 1 used.  Before each slice the
+     * wave waits for its own loads of slice s only, and the barrier makes the
+     * buffer complete for all waves and buffer (s + 3) % 4 free (every LDS
+     * read of slice s - 1 has been consumed by an MFMA before a wave gets
+     * there).  A plain s_barrier, not __syncthreads(): its workgroup fence
+     * would wait for all outstanding loads, the two prefetched slices
+     * included. */
+#define CURFP_DMA_SLICE(S, XA, XB, NA, NB)                                                                      \
+    wait_slice();                                                                                               \
+    __builtin_amdgcn_s_barrier();                                                                               \
+    mma_slice(acc, XA, XB, NA, NB, da, db, rsa, rsb, lda2, ldb2, ((S) + 3) * BK, K, wm, wn, wave, lane);
+    /* whole groups of four slices (a partial group multiplies zero slices:
+     * no data-dependent exit inside the unrolled body) */
+    for (int s = 0; s < nk; s += 4) {
+        CURFP_DMA_SLICE(s, sA0, sB0, sA3, sB3)
+        CURFP_DMA_SLICE(s + 1, sA1, sB1, sA0, sB0)
+        CURFP_DMA_SLICE(s + 2, sA2, sB2, sA1, sB1)
+        CURFP_DMA_SLICE(s + 3, sA3, sB3, sA2, sB2)
+    }
+#undef CURFP_DMA_SLICE
+    /* drain the zero-fill loads of the slices beyond K */
+    __builtin_amdgcn_s_waitcnt(0);
+
+    const bool use_beta = p.beta != 0.f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int64_t gj = n0 + wn * 128 + j * 32 + (lane & 31);
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gi = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
+                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
+                    float val = p.alpha * acc[i][j][r];
+                    if (use_beta) val += p.beta * (*cp);
+                    *cp = val;
+                }
+            }
+        }
+}
+
+}  // namespace
+
+/*
+ * Launch the DMA kernel for a BF16 row-block GEMM, or return
+ * hipErrorNotSupported when the operands do not meet its requirements
+ * (16-byte aligned rows and bases, K a multiple of 8, row offsets of a tile
+ * within 32 bits); the caller then uses the register-staged kernels.
+ */
+hipError_t launch_gemm_bf16_dma(const GemmArgs &g, hipStream_t s)
+{
+    if (g.M <= 0 || g.N <= 0) return hipSuccess;
+    if (!g.bf16_rows && !g.a_bf16) return hipErrorNotSupported;
+    if (g.va.cs != 1 || g.vb.cs != 1) return hipErrorNotSupported;
+    if ((g.va.rs % 8) || (g.vb.rs % 8) || (g.va.off % 8) || (g.vb.off % 8) || (g.K % 8)) return hipErrorNotSupported;
+    if (((uintptr_t)g.a % 16) || ((uintptr_t)g.b % 16)) return hipErrorNotSupported;
+    if (g.va.rs < g.K || g.vb.rs < g.K) return hipErrorNotSupported;
+    const int64_t lim = (int64_t)1 << 30;
+    if ((int64_t)BT * g.va.rs * 2 >= lim || (int64_t)BT * g.vb.rs * 2 >= lim || g.K >= lim)
+        return hipErrorNotSupported;
+    if (g.lower && g.M != g.N) return hipErrorInvalidValue;
+    DmaParams p;
+    p.a = (const unsigned short *)g.a + g.va.off; p.a_ld = g.va.rs;
+    p.b = (const unsigned short *)g.b + g.vb.off; p.b_ld = g.vb.rs;
+    p.c = (float *)g.c; p.c_off = g.vc.off; p.c_rs = g.vc.rs; p.c_cs = g.vc.cs;
+    p.M = g.M; p.N = g.N; p.K = g.K;
+    p.alpha = (float)g.alpha; p.beta = (float)g.beta;
+    p.lower = g.lower ? 1 : 0;
+    p.tiles_m = (g.M + BT - 1) / BT;
+    p.tiles_n = (g.N + BT - 1) / BT;
+    p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
+    if (p.ntiles <= 0) return hipSuccess;
+    if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gemm_bf16_dma_kernel, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    return hipGetLastError();
+}

@@ -50,6 +50,29 @@ def test_ssfrk_bf16_input_is_exact_products(curfp, t, u, n, k):
     assert np.linalg.norm(Cd - S) <= 2 * math.sqrt(k) * u32 * np.linalg.norm(S) + 1e-30
 
 
+@pytest.mark.parametrize("n,k,ld,t,u,beta", [(4500, 1000, 1000, "T", "U", 0.0), (4097, 520, 520, "N", "L", 0.5),
+                                              (4300, 1004, 1008, "T", "L", 0.0), (4100, 64, 64, "N", "U", -1.0)])
+def test_ssfrk_bf16_input_large_tiles(curfp, n, k, ld, t, u, beta):
+    """Sizes whose RFP blocks (>= 2048 rows) reach the 256 x 256 BF16 kernels:
+    direct-to-LDS loads when K % 8 == 0, the register-staged kernel otherwise
+    (k = 1004 inside rows of 1008); partial tiles at every edge, beta != 0.
+    Products of BF16 values are exact in FP32, so C matches the float64
+    result to FP32 accumulation accuracy."""
+    rng = np.random.default_rng(n + k)
+    A = (rng.standard_normal((n, ld)) / math.sqrt(k)).astype(np.float32)
+    Ab = torch.from_numpy(A).cuda().to(torch.bfloat16)
+    C0 = torch.from_numpy(rng.standard_normal(n * (n + 1) // 2).astype(np.float32)).cuda()
+    C = C0.clone()
+    curfp.ssfrk(Ab, C, alpha=0.75, beta=beta, transr=t, uplo=u, n=n, k=k, lda=ld)
+    A64 = Ab[:, :k].double()
+    S = (0.75 * (A64 @ A64.t())).cpu().numpy()
+    if beta != 0.0:
+        S += beta * O.rfp_to_dense_sym(C0.cpu().numpy().astype(np.float64), n, t, u)
+    Cd = O.rfp_to_dense_sym(C.cpu().numpy().astype(np.float64), n, t, u)
+    u32 = np.finfo(np.float32).eps
+    assert np.linalg.norm(Cd - S) <= 4 * math.sqrt(k) * u32 * np.linalg.norm(S)
+
+
 @pytest.mark.parametrize("t,u", VARIANTS)
 def test_ssfrk_math_modes(curfp, t, u):
     n, k = 777, 300
@@ -74,7 +97,8 @@ def test_ssfrk_math_modes(curfp, t, u):
 
 
 @pytest.mark.parametrize("mode", ["bf16", "bf16x3"])
-@pytest.mark.parametrize("n,t,u", [(2048, "T", "U"), (2047, "N", "L"), (1500, "N", "U"), (1501, "T", "L")])
+@pytest.mark.parametrize("n,t,u", [(2048, "T", "U"), (2047, "N", "L"), (1500, "N", "U"), (1501, "T", "L"),
+                                   (6001, "T", "U"), (5000, "N", "L")])
 def test_spftrf_math_modes_backward_error(curfp, mode, n, t, u):
     """Factor with BF16 / split-BF16 updates: ||L L^T - C|| / ||C|| at the
     precision of the mode (BF16 ~ 2^-8 scale, split-BF16 ~ FP32 scale)."""
