@@ -870,7 +870,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
     if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, false, s)) != hipSuccess) return st_of(e);
     it = 1;
-    double best = INFINITY;
+    double best_rn = INFINITY;
     for (;;) {
         /* R = B - C X (FP64), relative residual */
         e = gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, X, ldx, B, ldb, R, N,
@@ -882,12 +882,15 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
         if (e == hipSuccess) e = hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return st_of(e);
-        const double nx = sqrt(hs[1]);
-        rel = (normC > 0.0 && nx > 0.0) ? sqrt(hs[0]) / (normC * nx) : sqrt(hs[0]);
+        const double nx = sqrt(hs[1]), rn = sqrt(hs[0]);
+        rel = (normC > 0.0 && nx > 0.0) ? rn / (normC * nx) : rn;
         if (!(rel > tol) || it >= max_iter || !std::isfinite(rel)) break;
-        /* refinement diverging (factor too inaccurate for this conditioning): stop */
-        if (it >= 4 && rel > 4.0 * best) break;
-        best = std::min(best, rel);
+        /* Refinement diverging (the factor is too inaccurate for this
+         * conditioning: ||I - F^{-1} C|| > 1): stop while X is still finite.
+         * The test uses the absolute residual — the relative one stays
+         * bounded while X and R grow together. */
+        if (it >= 3 && rn > 2.0 * best_rn) break;
+        best_rn = std::min(best_rn, rn);
         /* D = F^{-1} fp32(R); X += D */
         if ((e = launch_cast_f64_f32(N, NR, R, N, D, N, s)) != hipSuccess) return st_of(e);
         if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);

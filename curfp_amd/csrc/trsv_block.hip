@@ -23,59 +23,85 @@ typedef float f32x4v __attribute__((ext_vector_type(4)));
 constexpr int LF = 128;        /* leaf */
 constexpr int MAXQ = 4;        /* leaves per block: 512 rows */
 constexpr int CW = 16;         /* right-hand-side columns per workgroup */
-constexpr int AP = LF + 4;     /* LDS pitch of the staged 128 x 128 operand */
+constexpr int AP = LF + 4;     /* LDS pitch of a staged operand held as As[i][k] */
+constexpr int ATP = LF + 16;   /* LDS pitch of a staged operand held as As[k][i] */
+constexpr int MAXSTEPS = MAXQ + MAXQ * (MAXQ - 1) / 2;
 constexpr int BP = CW + 1;     /* LDS pitch of the B slice */
 
-/*
- * Stage the 128 x 128 operand A(i, k) = base[off + i*rs + k*cs] (rows /
- * columns at or beyond `valid` read as zero) into As[i*AP + k], reading along
- * the contiguous direction of the source.
- */
-__device__ __forceinline__ void stage(float *As, const float *base, int64_t off, int64_t rs, int64_t cs, int valid_i,
-                                      int valid_k, int tid)
-{
-    constexpr int PER = LF * LF / 256;   /* 64 elements per thread, two rows / columns apart */
-    float v[PER];
-    /* all loads in flight before the first LDS store */
-    if (cs == 1) {
-        const int k = tid % LF, i0 = tid / LF;
-        const float *p = base + off + (int64_t)i0 * rs + k;
-        const int64_t step = 2 * rs;
-        const bool kv = k < valid_k;
+/* The 128 x 128 operand A(i, k) = base[off + i*rs + k*cs] of one step (rows /
+ * columns at or beyond vi / vk read as zero). */
+struct Blk {
+    const float *base;
+    int64_t off, rs, cs;
+    int vi, vk;
+};
+
+/* 16-byte vector that only assumes element alignment (RFP leading dimensions
+ * such as n + 1 leave rows unaligned) */
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+
+/* A thread's 64 elements of a staged operand as 16 vectors of 4 consecutive
+ * elements along the contiguous direction of the source (eight rows /
+ * columns apart). */
+struct Staged {
+    f32x4v v[16];
+
+    __device__ __forceinline__ void load(const Blk &b, int tid)
+    {
+        const int c4 = (tid % 32) * 4, o0 = tid / 32;
+        const bool kc = b.cs == 1;
+        /* kc: lanes run along k (c4 = k), o = i; else along i (c4 = i), o = k */
+        const int vc = kc ? b.vk : b.vi, vo = kc ? b.vi : b.vk;
+        const int64_t so = kc ? b.rs : b.cs;
+        const float *p = b.base + b.off + c4 + (int64_t)o0 * so;
+        const int64_t step = 8 * so;
+        const bool full = c4 + 4 <= vc;
 #pragma unroll
-        for (int e = 0; e < PER; ++e) v[e] = (kv && i0 + 2 * e < valid_i) ? p[e * step] : 0.f;
+        for (int e = 0; e < 16; ++e) {
+            const int o = o0 + 8 * e;
+            const float *q = p + e * step;
+            if (o < vo && full) {
+                const f32x4u x = *(const f32x4u *)q;
+                v[e] = f32x4v{x[0], x[1], x[2], x[3]};
+            } else {
 #pragma unroll
-        for (int e = 0; e < PER; ++e) As[(i0 + 2 * e) * AP + k] = v[e];
-    } else {
-        const int i = tid % LF, k0 = tid / LF;
-        const float *p = base + off + i + (int64_t)k0 * cs;
-        const int64_t step = 2 * cs;
-        const bool iv = i < valid_i;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) v[e] = (iv && k0 + 2 * e < valid_k) ? p[e * step] : 0.f;
-#pragma unroll
-        for (int e = 0; e < PER; ++e) As[i * AP + k0 + 2 * e] = v[e];
+                for (int c = 0; c < 4; ++c) v[e][c] = (o < vo && c4 + c < vc) ? q[c] : 0.f;
+            }
+        }
     }
-}
+    /* into LDS along the source's contiguous direction: As[i*AP + k] when k
+     * is contiguous (kc), else As[k*ATP + i] (both pitches keep the 16-byte
+     * stores aligned and the MFMA operand reads free of bank conflicts) */
+    __device__ __forceinline__ void store(float *As, bool kc, int tid) const
+    {
+        const int c4 = (tid % 32) * 4, o0 = tid / 32;
+        const int pitch = kc ? AP : ATP;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) *(f32x4v *)(As + (o0 + 8 * e) * pitch + c4) = v[e];
+    }
+};
 
 /*
  * acc (rows wave*32 .. +32, all 16 columns, as two 16 x 16 tiles) =
  * As (128 x 128) * Bs[q0 .. q0+128][0 .. 16)
  */
-__device__ __forceinline__ void mma_leaf(f32x4v (&acc)[2], const float *As, const float *Bs, int q0, int wave,
-                                         int lane)
+__device__ __forceinline__ void mma_leaf(f32x4v (&acc)[2], const float *As, bool kc, const float *Bs, int q0,
+                                         int wave, int lane)
 {
     const int li = lane & 15, lk = lane >> 4;
+    /* element (i, k) of the operand at As[i*ri + k*rk] */
+    const int ri = kc ? AP : 1, rk = kc ? 1 : ATP;
+    const float *a0 = As + (wave * 32 + li) * ri + lk * rk;
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[t][r] = 0.f;
-#pragma unroll 4
+#pragma unroll
     for (int k0 = 0; k0 < LF; k0 += 4) {
         const float b = Bs[(q0 + k0 + lk) * BP + li];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const float a = As[(wave * 32 + t * 16 + li) * AP + k0 + lk];
+            const float a = a0[t * 16 * ri + k0 * rk];
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc[t], 0, 0, 0);
         }
     }
@@ -100,7 +126,7 @@ template <bool TRANS>
 __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, RfpView Lv, int m, const float *inv,
                                                           float *Bbase, RfpView Bv, int nrhs)
 {
-    __shared__ float As[LF * AP];
+    __shared__ __attribute__((aligned(16))) float As[LF * ATP];
     __shared__ float Bs[MAXQ * LF * BP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int c0 = blockIdx.x * CW;
@@ -129,35 +155,70 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
     }
     __syncthreads();
 
-    for (int s = 0; s < nq; ++s) {
-        const int q = TRANS ? nq - 1 - s : s;
-        const int mq = min(LF, m - q * LF);
-        /* Y_q = inv(L_qq) B_q   (backward: inv(L_qq)^T B_q) */
-        const float *W = inv + (int64_t)q * LF * LF;
-        if (TRANS) stage(As, W, 0, 1, LF, mq, mq, tid);
-        else       stage(As, W, 0, LF, 1, mq, mq, tid);
-        __syncthreads();
-        f32x4v acc[2];
-        mma_leaf(acc, As, Bs, q * LF, wave, lane);
-        __syncthreads();
-        put_leaf(acc, Bs, q * LF, 1.f, false, wave, lane);
-        __syncthreads();
-        /* the other leaves: forward r > q: B_r -= L_rq Y_q; backward r < q: B_r -= L_qr^T X_q */
-        const int r_lo = TRANS ? 0 : q + 1, r_hi = TRANS ? q : nq;
-        for (int r = r_lo; r < r_hi; ++r) {
-            const int mr = min(LF, m - r * LF);
-            if (TRANS) {
-                /* A(i, k) = L(q*LF + k, r*LF + i) */
-                const RfpView blk = Lv.sub(q * LF, r * LF);
-                stage(As, Lbase, blk.off, blk.cs, blk.rs, mr, mq, tid);
-            } else {
-                const RfpView blk = Lv.sub(r * LF, q * LF);
-                stage(As, Lbase, blk.off, blk.rs, blk.cs, mr, mq, tid);
+    /*
+     * The block's steps in order — forward: for each leaf q, Y_q = inv(L_qq) B_q,
+     * then B_r -= L_rq Y_q for r > q; backward: for q from the last leaf,
+     * X_q = inv(L_qq)^T B_q, then B_r -= L_qr^T X_q for r < q.
+     */
+    auto step_of = [&](int j, int &q, int &r) {
+        /* j -> (q, r): r == q for the leaf inverse, else the update of leaf r */
+        for (int s = 0; s < nq; ++s) {
+            const int qq = TRANS ? nq - 1 - s : s;
+            const int nr = TRANS ? qq : nq - 1 - qq;   /* updates after this leaf */
+            if (j <= nr) {
+                q = qq;
+                r = j == 0 ? qq : (TRANS ? j - 1 : qq + j);
+                return;
             }
+            j -= nr + 1;
+        }
+        q = r = 0;
+    };
+    auto blk_of = [&](int q, int r) {
+        const int mq = min(LF, m - q * LF), mr = min(LF, m - r * LF);
+        if (r == q) {
+            const float *W = inv + (int64_t)q * LF * LF;
+            return TRANS ? Blk{W, 0, 1, LF, mq, mq} : Blk{W, 0, LF, 1, mq, mq};
+        }
+        if (TRANS) {
+            /* A(i, k) = L(q*LF + k, r*LF + i) */
+            const RfpView v = Lv.sub(q * LF, r * LF);
+            return Blk{Lbase, v.off, v.cs, v.rs, mr, mq};
+        }
+        const RfpView v = Lv.sub(r * LF, q * LF);
+        return Blk{Lbase, v.off, v.rs, v.cs, mr, mq};
+    };
+    const int nsteps = nq + nq * (nq - 1) / 2;
+    /* two register sets: set j % 2 holds step j's operand; steps j + 1 and
+     * j + 2 are in flight while step j's MFMAs run */
+    Staged st[2];
+    Blk bk[2];
+    int qs[2], rs[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        if (j < nsteps) {
+            step_of(j, qs[j], rs[j]);
+            bk[j] = blk_of(qs[j], rs[j]);
+            st[j].load(bk[j], tid);
+        }
+#pragma unroll
+    for (int j = 0; j < MAXSTEPS; ++j) {
+        if (j < nsteps) {
+            const int x = j & 1;
+            const bool kc = bk[x].cs == 1;
+            const int cq = qs[x], cr = rs[x];
+            st[x].store(As, kc, tid);
             __syncthreads();
-            mma_leaf(acc, As, Bs, q * LF, wave, lane);
+            if (j + 2 < nsteps) {
+                step_of(j + 2, qs[x], rs[x]);
+                bk[x] = blk_of(qs[x], rs[x]);
+                st[x].load(bk[x], tid);
+            }
+            f32x4v acc[2];
+            mma_leaf(acc, As, kc, Bs, cq * LF, wave, lane);
             __syncthreads();
-            put_leaf(acc, Bs, r * LF, -1.f, true, wave, lane);
+            if (cr == cq) put_leaf(acc, Bs, cq * LF, 1.f, false, wave, lane);
+            else          put_leaf(acc, Bs, cr * LF, -1.f, true, wave, lane);
             __syncthreads();
         }
     }

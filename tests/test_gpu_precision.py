@@ -173,3 +173,23 @@ def test_refinement_to_fp64_accuracy(curfp):
     Xref = np.linalg.solve(S, B.astype(np.float64))
     err = np.linalg.norm(X.cpu().numpy() - Xref) / np.linalg.norm(Xref)
     assert info["rel_residual"] <= 1e-14 and err <= 1e-12, (info, err)
+
+
+def test_refinement_stops_when_diverging(curfp):
+    """A factor of C~ = A A^T + 0.01 I used to refine C = A A^T + 0.05 I with
+    rank(A A^T) < n: I - C~^{-1} C has eigenvalue -(0.05 - 0.01)/0.01 = -4 on
+    the null space of A^T, so the iteration diverges.  The driver stops after
+    a few solves with the iterate still finite and reports the residual."""
+    n, k, nrhs = 1200, 100, 4
+    A, B = problem(n, k, nrhs, seed=12)
+    Ad = torch.from_numpy(A).cuda()
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(Ad, C)
+    curfp.add_to_diagonal(C, 0.01)
+    assert curfp.spftrf(C) == 0
+    from curfp_amd import refine
+
+    X, info = refine.refine_solve(C, Ad, 0.05, torch.from_numpy(B).cuda(), tol=1e-6, max_iter=40)
+    assert info["solves"] <= 6, info
+    assert info["rel_residual"] > 1e-6
+    assert bool(torch.isfinite(X).all())
