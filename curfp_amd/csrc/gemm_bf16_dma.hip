@@ -10,20 +10,18 @@
  *
  * Structure: 256 x 256 output tile, 4 wave64s (2 x 2) of 128 x 128 each —
  * 4 x 4 blocks of v_mfma_f32_32x32x16_bf16, 256 FP32 accumulators per lane.
- * k-slices of BK = 64 travel global -> LDS by buffer_load_dwordx4 ... lds
+ * k-slices of BK = 32 travel global -> LDS by buffer_load_dwordx4 ... lds
  * (no VGPR staging, no LDS store instructions): one wave instruction fills
- * 1 KB of LDS (8 rows of 128 bytes) with lane l's 16 bytes at offset 16 l,
+ * 1 KB of LDS (16 rows of 64 bytes) with lane l's 16 bytes at offset 16 l,
  * and each lane chooses which global 16-byte chunk it fetches so that LDS
- * chunk p of row r holds global chunk p ^ ((r >> 1) & 7).  A 128-byte row
- * spans half of the 64 LDS banks, so with that XOR swizzle the 16 lanes of
- * one LDS cycle (16 consecutive rows, one chunk each) hit all 64 banks once.  Per-tile buffer descriptors bound the rows (num_records covers
+ * chunk p of row r holds global chunk p ^ ((r >> 2) & 3).  A 64-byte row
+ * spans a quarter of the 64 LDS banks, so with that XOR swizzle the 16 lanes
+ * of one LDS cycle (16 consecutive rows, one chunk each) hit all 64 banks
+ * once.  Per-tile buffer descriptors bound the rows (num_records covers
  * exactly the valid rows) and chunks at or beyond K get an offset outside
  * the descriptor, so the edges of the problem read as zeros without
- * branches.  Two LDS buffers per operand (128 KB in all) alternate with the
- * loop unrolled by two, so each buffer is a distinct LDS object and the next
- * slice's loads stay in flight during the current slice's MFMAs; operand
- * fragments are double-buffered in registers across the four 16-wide k
- * steps of a slice.
+ * branches.  Four LDS buffers (128 KB in all) keep three slices in flight;
+ * the loop is unrolled by four so each buffer is a distinct LDS object.
  */
 #include "rfp_internal.h"
 
