@@ -482,12 +482,10 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
         const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) < 128;
         const bool big = g.M >= 2048 && g.N >= 2048;
         if (x3) return small ? launch_rows<64, true, 64>(g, s) : launch_rows<128, true, 32>(g, s);
-        if (big) {
-            const hipError_t e = launch_gemm_bf16_dma(g, s);
-            if (e != hipErrorNotSupported) return e;
-            return launch_rows<256, false, 64>(g, s);
-        }
-        return small ? launch_rows<64, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
+        if (small) return launch_rows<64, false, 64>(g, s);
+        const hipError_t e = launch_gemm_bf16_dma(g, big ? 256 : 128, s);
+        if (e != hipErrorNotSupported) return e;
+        return big ? launch_rows<256, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
     }
     BKParams p;
     p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;

@@ -8,8 +8,9 @@
  * (the factor's BF16 shadow, or the caller's BF16 matrix); C is an FP32 view
  * into the RFP array.
  *
- * Structure: 256 x 256 output tile, 4 wave64s (2 x 2) of 128 x 128 each —
- * 4 x 4 blocks of v_mfma_f32_32x32x16_bf16, 256 FP32 accumulators per lane.
+ * Structure: 256 x 256 output tile (128 x 128 for mid-size problems), 4
+ * wave64s (2 x 2) of 128 x 128 (64 x 64) each — 4 x 4 (2 x 2) blocks of
+ * v_mfma_f32_32x32x16_bf16, 256 (64) FP32 accumulators per lane.
  * k-slices of BK = 32 travel global -> LDS by buffer_load_dwordx4 ... lds
  * (no VGPR staging, no LDS store instructions): one wave instruction fills
  * 1 KB of LDS (16 rows of 64 bytes) with lane l's 16 bytes at offset 16 l,
@@ -20,7 +21,7 @@
  * once.  Per-tile buffer descriptors bound the rows (num_records covers
  * exactly the valid rows) and chunks at or beyond K get an offset outside
  * the descriptor, so the edges of the problem read as zeros without
- * branches.  Four LDS buffers (128 KB in all) keep three slices in flight;
+ * branches.  Four LDS buffers (128 KB in all at 256 x 256) keep three slices in flight;
  * the loop is unrolled by four so each buffer is a distinct LDS object.
  */
 #include "rfp_internal.h"
@@ -31,11 +32,16 @@ typedef float  f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
-constexpr int BT = 256;              /* tile rows / columns */
 constexpr int BK = 32;               /* k per slice */
 constexpr int ROWB = BK * 2;         /* bytes of one row of a slice: 4 chunks of 16 */
-constexpr int SLICE = BT * ROWB;     /* bytes of one operand slice: 16 KB */
-constexpr int NBUF = 4;              /* LDS buffers: slices s .. s + 3 */
+
+/* Tile BT x BT (256 or 128): 4 waves of BT/2 x BT/2, F x F blocks of 32 x 32
+ * each; an operand slice is BT rows of ROWB bytes, loaded by F wave
+ * instructions per wave (16 rows each). */
+template <int BT> struct Tile {
+    static constexpr int F = BT / 64;
+    static constexpr int SLICE = BT * ROWB;
+};
 constexpr int RSRC_FLAGS = 0x00020000;
 
 struct DmaParams {
@@ -97,8 +103,8 @@ struct SliceDma {
 
 /* Operand fragments of one 16-wide k step t of a slice: lane l reads row
  * l % 32 of each 32-row block, global chunk 2 t + l / 32 (swizzled). */
-struct Frags {
-    bf16x8 a[4], b[4];
+template <int F> struct Frags {
+    bf16x8 a[F], b[F];
 };
 
 __device__ __forceinline__ int frag_off(int t, int lane)
@@ -115,20 +121,22 @@ __device__ __forceinline__ int frag_off(int t, int lane)
  * read follows a global -> LDS load within a slice, so the compiler has no
  * reason to wait on the loads in flight.
  */
-__device__ __forceinline__ void mma_slice(f32x16 (&acc)[4][4], const unsigned char *As, const unsigned char *Bs,
-                                          unsigned char *Na, unsigned char *Nb, const SliceDma &da,
-                                          const SliceDma &db, __amdgpu_buffer_rsrc_t rsa,
+template <int BT>
+__device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F], const unsigned char *As,
+                                          const unsigned char *Bs, unsigned char *Na, unsigned char *Nb,
+                                          const SliceDma &da, const SliceDma &db, __amdgpu_buffer_rsrc_t rsa,
                                           __amdgpu_buffer_rsrc_t rsb, int lda2, int ldb2, int k_next, int K,
                                           int wm, int wn, int wave, int lane)
 {
-    const unsigned char *arow = As + wm * 128 * ROWB;
-    const unsigned char *brow = Bs + wn * 128 * ROWB;
-    Frags f[2];
+    constexpr int F = Tile<BT>::F;
+    const unsigned char *arow = As + wm * (BT / 2) * ROWB;
+    const unsigned char *brow = Bs + wn * (BT / 2) * ROWB;
+    Frags<F> f[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
         const int o = frag_off(t, lane);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < F; ++i) {
             f[t].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
             f[t].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
         }
@@ -137,29 +145,33 @@ __device__ __forceinline__ void mma_slice(f32x16 (&acc)[4][4], const unsigned ch
 #pragma unroll
     for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
+        for (int i = 0; i < F; ++i) {
             if (t == 0) {
                 da.issue(Na, rsa, lda2, k_next, K, i, wave);
                 db.issue(Nb, rsb, ldb2, k_next, K, i, wave);
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < F; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[t].a[i], f[t].b[j], acc[i][j], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
 }
 
-/* wait until at most 16 of this wave's global -> LDS loads (two slices: 4 A +
- * 4 B instructions each) are outstanding, i.e. the oldest slice has landed;
+/* wait until at most 4 F of this wave's global -> LDS loads (two slices: F A +
+ * F B instructions each) are outstanding, i.e. the oldest slice has landed;
  * lgkm and exp counters are not waited on */
+template <int F>
 __device__ __forceinline__ void wait_slice()
 {
-    __builtin_amdgcn_s_waitcnt((1 << 14) | (0xF << 8) | (0x7 << 4) | 0x0);
+    constexpr int vm = 4 * F;
+    __builtin_amdgcn_s_waitcnt(((vm >> 4) << 14) | (0xF << 8) | (0x7 << 4) | (vm & 0xF));
 }
 
+template <int BT>
 __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
 {
+    constexpr int F = Tile<BT>::F, SLICE = Tile<BT>::SLICE;
     /* one LDS object per buffer (A slice, then B slice) */
     __shared__ __attribute__((aligned(1024))) unsigned char s0[2 * SLICE];
     __shared__ __attribute__((aligned(1024))) unsigned char s1[2 * SLICE];
@@ -188,11 +200,11 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
         __builtin_amdgcn_make_buffer_rsrc((void *)(p.b + n0 * p.b_ld), 0, (int)(rb * ldb2), RSRC_FLAGS);
     const int K = (int)p.K;
 
-    f32x16 acc[4][4];
+    f32x16 acc[F][F];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < F; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < F; ++j)
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
@@ -204,17 +216,17 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
         /* slices 0, 1, 2 into buffers 0, 1, 2 (a slice at or beyond K loads
          * as zeros) */
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < F; ++q) {
             da.issue(sA0, rsa, lda2, 0, K, q, wave);
             db.issue(sB0, rsb, ldb2, 0, K, q, wave);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < F; ++q) {
             da.issue(sA1, rsa, lda2, BK, K, q, wave);
             db.issue(sB1, rsb, ldb2, BK, K, q, wave);
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < F; ++q) {
             da.issue(sA2, rsa, lda2, 2 * BK, K, q, wave);
             db.issue(sB2, rsb, ldb2, 2 * BK, K, q, wave);
         }
@@ -229,9 +241,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
      * would wait for all outstanding loads, the two prefetched slices
      * included. */
 #define CURFP_DMA_SLICE(S, XA, XB, NA, NB)                                                                      \
-    wait_slice();                                                                                               \
+    wait_slice<F>();                                                                                            \
     __builtin_amdgcn_s_barrier();                                                                               \
-    mma_slice(acc, XA, XB, NA, NB, da, db, rsa, rsb, lda2, ldb2, ((S) + 3) * BK, K, wm, wn, wave, lane);
+    mma_slice<BT>(acc, XA, XB, NA, NB, da, db, rsa, rsb, lda2, ldb2, ((S) + 3) * BK, K, wm, wn, wave, lane);
     /* whole groups of four slices (a partial group multiplies zero slices:
      * no data-dependent exit inside the unrolled body) */
     for (int s = 0; s < nk; s += 4) {
@@ -246,13 +258,13 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
 
     const bool use_beta = p.beta != 0.f;
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < F; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const int64_t gj = n0 + wn * 128 + j * 32 + (lane & 31);
+        for (int j = 0; j < F; ++j) {
+            const int64_t gj = n0 + wn * (BT / 2) + j * 32 + (lane & 31);
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * 128 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int64_t gi = m0 + wm * (BT / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
                 if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
                     float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
                     float val = p.alpha * acc[i][j][r];
@@ -271,8 +283,9 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
  * (16-byte aligned rows and bases, K a multiple of 8, row offsets of a tile
  * within 32 bits); the caller then uses the register-staged kernels.
  */
-hipError_t launch_gemm_bf16_dma(const GemmArgs &g, hipStream_t s)
+hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s)
 {
+    if (bt != 128 && bt != 256) return hipErrorNotSupported;
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     if (!g.bf16_rows && !g.a_bf16) return hipErrorNotSupported;
     if (g.va.cs != 1 || g.vb.cs != 1) return hipErrorNotSupported;
@@ -280,7 +293,7 @@ hipError_t launch_gemm_bf16_dma(const GemmArgs &g, hipStream_t s)
     if (((uintptr_t)g.a % 16) || ((uintptr_t)g.b % 16)) return hipErrorNotSupported;
     if (g.va.rs < g.K || g.vb.rs < g.K) return hipErrorNotSupported;
     const int64_t lim = (int64_t)1 << 30;
-    if ((int64_t)BT * g.va.rs * 2 >= lim || (int64_t)BT * g.vb.rs * 2 >= lim || g.K >= lim)
+    if ((int64_t)bt * g.va.rs * 2 >= lim || (int64_t)bt * g.vb.rs * 2 >= lim || g.K >= lim)
         return hipErrorNotSupported;
     if (g.lower && g.M != g.N) return hipErrorInvalidValue;
     DmaParams p;
@@ -290,11 +303,12 @@ hipError_t launch_gemm_bf16_dma(const GemmArgs &g, hipStream_t s)
     p.M = g.M; p.N = g.N; p.K = g.K;
     p.alpha = (float)g.alpha; p.beta = (float)g.beta;
     p.lower = g.lower ? 1 : 0;
-    p.tiles_m = (g.M + BT - 1) / BT;
-    p.tiles_n = (g.N + BT - 1) / BT;
+    p.tiles_m = (g.M + bt - 1) / bt;
+    p.tiles_n = (g.N + bt - 1) / bt;
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
     if (p.ntiles <= 0) return hipSuccess;
     if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(gemm_bf16_dma_kernel, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    if (bt == 256) hipLaunchKernelGGL(gemm_bf16_dma_kernel<256>, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    else           hipLaunchKernelGGL(gemm_bf16_dma_kernel<128>, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
     return hipGetLastError();
 }

@@ -200,9 +200,10 @@ struct GemmArgs {
 };
 static inline GemmArgs gemm_args() { GemmArgs g{}; g.splits = 1; return g; }
 hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s);
-/* 256 x 256 BF16 row-block GEMM with direct global -> LDS loads; returns
- * hipErrorNotSupported when the operands do not meet its requirements */
-hipError_t launch_gemm_bf16_dma(const GemmArgs &g, hipStream_t s);
+/* BF16 row-block GEMM with direct global -> LDS loads, bt x bt tiles (256 or
+ * 128); returns hipErrorNotSupported when the operands do not meet its
+ * requirements */
+hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s);
 
 /* Largest N (inplace = 1) / M (inplace = 2) the in-place GEMM supports. */
