@@ -11,18 +11,20 @@
  * Structure: 256 x 256 output tile (128 x 128 for mid-size problems), 4
  * wave64s (2 x 2) of 128 x 128 (64 x 64) each — 4 x 4 (2 x 2) blocks of
  * v_mfma_f32_32x32x16_bf16, 256 (64) FP32 accumulators per lane.
- * k-slices of BK = 32 travel global -> LDS by buffer_load_dwordx4 ... lds
+ * k-slices of BK = 64 travel global -> LDS by buffer_load_dwordx4 ... lds
  * (no VGPR staging, no LDS store instructions): one wave instruction fills
- * 1 KB of LDS (16 rows of 64 bytes) with lane l's 16 bytes at offset 16 l,
- * and each lane chooses which global 16-byte chunk it fetches so that LDS
- * chunk p of row r holds global chunk p ^ ((r >> 2) & 3).  A 64-byte row
- * spans a quarter of the 64 LDS banks, so with that XOR swizzle the 16 lanes
- * of one LDS cycle (16 consecutive rows, one chunk each) hit all 64 banks
- * once.  Per-tile buffer descriptors bound the rows (num_records covers
+ * 1 KB of LDS (8 rows of 128 bytes, whole cache lines) with lane l's 16
+ * bytes at offset 16 l, and each lane chooses which global 16-byte chunk it
+ * fetches so that LDS chunk p of row r holds global chunk p ^ ((r >> 1) & 7).
+ * A 128-byte row spans half of the 64 LDS banks, so with that XOR swizzle
+ * the 16 lanes of one LDS cycle (16 consecutive rows, one chunk each) hit
+ * all 64 banks once.  Per-tile buffer descriptors bound the rows (num_records covers
  * exactly the valid rows) and chunks at or beyond K get an offset outside
  * the descriptor, so the edges of the problem read as zeros without
- * branches.  Four LDS buffers (128 KB in all at 256 x 256) keep three slices in flight;
- * the loop is unrolled by four so each buffer is a distinct LDS object.
+ * branches.  Two LDS buffers (128 KB in all at 256 x 256) alternate with the
+ * loop unrolled by two, so each buffer is a distinct LDS object and the
+ * compiler does not wait on the next slice's loads before reading the
+ * current one.
  */
 #include "rfp_internal.h"
 
@@ -32,12 +34,12 @@ typedef float  f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 #define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
-constexpr int BK = 32;               /* k per slice */
-constexpr int ROWB = BK * 2;         /* bytes of one row of a slice: 4 chunks of 16 */
+constexpr int BK = 64;               /* k per slice */
+constexpr int ROWB = BK * 2;         /* bytes of one row of a slice: 8 chunks of 16 */
 
 /* Tile BT x BT (256 or 128): 4 waves of BT/2 x BT/2, F x F blocks of 32 x 32
- * each; an operand slice is BT rows of ROWB bytes, loaded by F wave
- * instructions per wave (16 rows each). */
+ * each; an operand slice is BT rows of ROWB bytes, loaded by 2 F wave
+ * instructions per wave (8 rows each). */
 template <int BT> struct Tile {
     static constexpr int F = BT / 64;
     static constexpr int SLICE = BT * ROWB;
@@ -78,17 +80,19 @@ __device__ __forceinline__ void tile_coords(const DmaParams &p, int64_t t, int64
     }
 }
 
-/* Loads of one operand's k-slice [k0, k0 + BK): 16 groups of 16 rows, 4 per
- * wave (q = 0..3).  Lane l of group g handles row 16 g + l / 4 and LDS chunk
- * l % 4, which holds global chunk (l % 4) ^ ((row >> 2) & 3) = (l % 4) ^ (l / 16). */
+/* Loads of one operand's k-slice [k0, k0 + BK): groups of 8 rows, q-th of a
+ * wave = group 4 q + wave.  Lane l handles row 8 g + l / 8 and LDS chunk
+ * l % 8, which holds global chunk (l % 8) ^ ((row >> 1) & 7). */
 struct SliceDma {
-    int rowoff;   /* (l / 4) * ld2: byte offset of the lane's row within a group */
+    int rowoff;   /* (l / 8) * ld2: byte offset of the lane's row within a group */
     int c8;       /* 8 * global chunk of the lane */
 
-    __device__ __forceinline__ void init(int ld2, int lane)
+    __device__ __forceinline__ void init(int ld2, int lane, int wave)
     {
-        c8 = 8 * ((lane & 3) ^ (lane >> 4));
-        rowoff = (lane >> 2) * ld2;
+        const int rl = lane >> 3;
+        /* row 8 g + rl with g = 4 q + wave: (row >> 1) & 7 == 4 (wave & 1) + rl / 2 */
+        c8 = 8 * ((lane & 7) ^ ((4 * (wave & 1) + (rl >> 1)) & 7));
+        rowoff = rl * ld2;
     }
     __device__ __forceinline__ void issue(unsigned char *lds, __amdgpu_buffer_rsrc_t rs, int ld2, int k0, int K,
                                           int q, int wave) const
@@ -96,7 +100,7 @@ struct SliceDma {
         const int g = q * 4 + wave;
         const int kk = k0 + c8;
         /* K % 8 == 0: a chunk is wholly inside or outside [0, K) */
-        const int off = kk < K ? 16 * g * ld2 + rowoff + kk * 2 : (int)0x80000000;
+        const int off = kk < K ? 8 * g * ld2 + rowoff + kk * 2 : (int)0x80000000;
         __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, LDS_PTR(lds + g * 1024), 16, off, 0, 0, 0);
     }
 };
@@ -109,17 +113,16 @@ template <int F> struct Frags {
 
 __device__ __forceinline__ int frag_off(int t, int lane)
 {
-    /* row = 32 i + (lane & 31) (+ 128 wm): (row >> 2) & 3 == ((lane & 31) >> 2) & 3 */
-    return (lane & 31) * ROWB + ((((2 * t) + (lane >> 5)) ^ (((lane & 31) >> 2) & 3)) << 4);
+    /* row = 32 i + (lane & 31) (+ BT/2 wm): (row >> 1) & 7 == ((lane & 31) >> 1) & 7 */
+    return (lane & 31) * ROWB + ((((2 * t) + (lane >> 5)) ^ (((lane & 31) >> 1) & 7)) << 4);
 }
 
 /*
- * acc += the wave's 128 x 128 block of the slice in (As, Bs), two 16-wide k
- * steps.  All 16 operand reads of the slice are issued first; the loads of
- * slice s + 3 (into Na, Nb) are then issued between the groups of four
- * MFMAs of step 0 (the order is fixed with scheduling barriers).  No LDS
- * read follows a global -> LDS load within a slice, so the compiler has no
- * reason to wait on the loads in flight.
+ * acc += the wave's block of the slice in (As, Bs): four 16-wide k steps,
+ * operand fragments double-buffered in registers (step t + 1's read two at a
+ * time between step t's groups of F MFMAs), and during step 0 the loads of
+ * the next slice (into Na, Nb) issued the same way.  The issue order is fixed
+ * with scheduling barriers.
  */
 template <int BT>
 __device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F], const unsigned char *As,
@@ -132,40 +135,38 @@ __device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F
     const unsigned char *arow = As + wm * (BT / 2) * ROWB;
     const unsigned char *brow = Bs + wn * (BT / 2) * ROWB;
     Frags<F> f[2];
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-        const int o = frag_off(t, lane);
+    {
+        const int o = frag_off(0, lane);
 #pragma unroll
         for (int i = 0; i < F; ++i) {
-            f[t].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
-            f[t].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
+            f[0].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
+            f[0].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
         }
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < 4; ++t) {
+        const int cur = t & 1, nxt = cur ^ 1;
+        const int o = frag_off(t + 1, lane);
 #pragma unroll
         for (int i = 0; i < F; ++i) {
+            if (t < 3) {
+                f[nxt].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
+                f[nxt].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
+            }
             if (t == 0) {
-                da.issue(Na, rsa, lda2, k_next, K, i, wave);
-                db.issue(Nb, rsb, ldb2, k_next, K, i, wave);
+                da.issue(Na, rsa, lda2, k_next, K, 2 * i, wave);
+                da.issue(Na, rsa, lda2, k_next, K, 2 * i + 1, wave);
+                db.issue(Nb, rsb, ldb2, k_next, K, 2 * i, wave);
+                db.issue(Nb, rsb, ldb2, k_next, K, 2 * i + 1, wave);
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int j = 0; j < F; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[t].a[i], f[t].b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[cur].a[i], f[cur].b[j], acc[i][j], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
-}
-
-/* wait until at most 4 F of this wave's global -> LDS loads (two slices: F A +
- * F B instructions each) are outstanding, i.e. the oldest slice has landed;
- * lgkm and exp counters are not waited on */
-template <int F>
-__device__ __forceinline__ void wait_slice()
-{
-    constexpr int vm = 4 * F;
-    __builtin_amdgcn_s_waitcnt(((vm >> 4) << 14) | (0xF << 8) | (0x7 << 4) | (vm & 0xF));
+    }
 }
 
 template <int BT>
@@ -175,12 +176,8 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
     /* one LDS object per buffer (A slice, then B slice) */
     __shared__ __attribute__((aligned(1024))) unsigned char s0[2 * SLICE];
     __shared__ __attribute__((aligned(1024))) unsigned char s1[2 * SLICE];
-    __shared__ __attribute__((aligned(1024))) unsigned char s2[2 * SLICE];
-    __shared__ __attribute__((aligned(1024))) unsigned char s3[2 * SLICE];
     unsigned char *const sA0 = s0, *const sB0 = s0 + SLICE;
     unsigned char *const sA1 = s1, *const sB1 = s1 + SLICE;
-    unsigned char *const sA2 = s2, *const sB2 = s2 + SLICE;
-    unsigned char *const sA3 = s3, *const sB3 = s3 + SLICE;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
@@ -209,51 +206,33 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
             for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
 
     SliceDma da, db;
-    da.init(lda2, lane);
-    db.init(ldb2, lane);
+    da.init(lda2, lane, wave);
+    db.init(ldb2, lane, wave);
     const int nk = (K + BK - 1) / BK;
     if (nk > 0) {
-        /* slices 0, 1, 2 into buffers 0, 1, 2 (a slice at or beyond K loads
-         * as zeros) */
 #pragma unroll
-        for (int q = 0; q < F; ++q) {
+        for (int q = 0; q < 2 * F; ++q) {
             da.issue(sA0, rsa, lda2, 0, K, q, wave);
             db.issue(sB0, rsb, ldb2, 0, K, q, wave);
         }
-#pragma unroll
-        for (int q = 0; q < F; ++q) {
-            da.issue(sA1, rsa, lda2, BK, K, q, wave);
-            db.issue(sB1, rsb, ldb2, BK, K, q, wave);
-        }
-#pragma unroll
-        for (int q = 0; q < F; ++q) {
-            da.issue(sA2, rsa, lda2, 2 * BK, K, q, wave);
-            db.issue(sB2, rsb, ldb2, 2 * BK, K, q, wave);
-        }
     }
-    /* Slice s sits in buffer s % 4 and is multiplied while slices s + 1 and
-     * s + 2 are in flight; its step 0 issues the loads of slice s + 3 into
-     * buffer (s + 3) % 4, the one slice s - 1 used.  Before each slice the
-     * wave waits for its own loads of slice s only, and the barrier makes the
-     * buffer complete for all waves and buffer (s + 3) % 4 free (every LDS
-     * read of slice s - 1 has been consumed by an MFMA before a wave gets
-     * there).  A plain s_barrier, not __syncthreads(): its workgroup fence
-     * would wait for all outstanding loads, the two prefetched slices
-     * included. */
-#define CURFP_DMA_SLICE(S, XA, XB, NA, NB)                                                                      \
-    wait_slice<F>();                                                                                            \
-    __builtin_amdgcn_s_barrier();                                                                               \
-    mma_slice<BT>(acc, XA, XB, NA, NB, da, db, rsa, rsb, lda2, ldb2, ((S) + 3) * BK, K, wm, wn, wave, lane);
-    /* whole groups of four slices (a partial group multiplies zero slices:
-     * no data-dependent exit inside the unrolled body) */
-    for (int s = 0; s < nk; s += 4) {
-        CURFP_DMA_SLICE(s, sA0, sB0, sA3, sB3)
-        CURFP_DMA_SLICE(s + 1, sA1, sB1, sA0, sB0)
-        CURFP_DMA_SLICE(s + 2, sA2, sB2, sA1, sB1)
-        CURFP_DMA_SLICE(s + 3, sA3, sB3, sA2, sB2)
+    /* Slice s sits in buffer s % 2; its step 0 issues the loads of slice
+     * s + 1 into the other buffer (a slice at or beyond K loads as zeros;
+     * an odd slice count multiplies one zero slice, so the body unrolled by
+     * two has no data-dependent exit).  Before each slice the wave waits for
+     * its own loads, and the barrier makes the buffer complete for all waves
+     * and the other buffer free (every LDS read of the previous slice has
+     * been consumed by an MFMA before a wave gets there).  A plain
+     * s_barrier: nothing else is outstanding. */
+    for (int s = 0; s < nk; s += 2) {
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_barrier();
+        mma_slice<BT>(acc, sA0, sB0, sA1, sB1, da, db, rsa, rsb, lda2, ldb2, (s + 1) * BK, K, wm, wn, wave, lane);
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_s_barrier();
+        mma_slice<BT>(acc, sA1, sB1, sA0, sB0, da, db, rsa, rsb, lda2, ldb2, (s + 2) * BK, K, wm, wn, wave, lane);
     }
-#undef CURFP_DMA_SLICE
-    /* drain the zero-fill loads of the slices beyond K */
+    /* drain the zero-fill loads of the slice beyond K */
     __builtin_amdgcn_s_waitcnt(0);
 
     const bool use_beta = p.beta != 0.f;
