@@ -191,7 +191,8 @@ def test_config3_precision(curfp):
                solves=it["solves"], rel_residual=it["rel_residual"])
         if it["rel_residual"] <= 1e-6:
             # kappa(C) ~ 3e3 here: a 1e-6 residual bounds the forward error to ~1e-2
-            assert err <= 5e-2 and err <= err_plain
+            # refinement never makes the error worse (equal when one solve already met the tolerance)
+            assert err <= 5e-2 and err <= err_plain * (1 + 1e-9)
 
 
 def test_config4_backward_cost(curfp):
