@@ -111,8 +111,8 @@ def run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, tol, max_iter, ev=None):
 
 KERNEL_INFO = {
     "gemm_f32": ("view_gemm_kernel<float> (FP32 MFMA v_mfma_f32_32x32x2f32)", "fp32"),
-    "gemm_bf16": ("gemm_bf16_rows_kernel / view_gemm_bf16_kernel (BF16 MFMA v_mfma_f32_32x32x16_bf16, "
-                  "FP32 accumulate)", "bf16"),
+    "gemm_bf16": ("gemm_bf16_dma_kernel<256|128> (+ gemm_bf16_rows_kernel<64> for small updates; BF16 MFMA "
+                  "v_mfma_f32_32x32x16_bf16, FP32 accumulate, direct global->LDS loads)", "bf16"),
     "gemm_f64": ("view_gemm_kernel<double> (FP64 MFMA v_mfma_f64_16x16x4f64)", "fp64"),
     "potrf_tile": ("potrf_leaf_kernel (128-row Cholesky leaf + its inverse, FP32 MFMA)", "fp32"),
     "trsv": ("block_solve_kernel (512-row block substitution, FP32 MFMA)", "fp32"),
