@@ -63,12 +63,21 @@ extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
     ctx->d_info = nullptr;
+    ctx->sweep_sync = nullptr;
+    ctx->sweep_epoch = 0;
     ctx->prof = false;
     ctx->sh = nullptr;
     ctx->sh_bytes = 0;
     for (RfpKernelStat &st : ctx->prof_stats) st = RfpKernelStat{0, 0.0, 0.0, 0.0};
     if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return CURFP_STATUS_NOT_INITIALIZED; }
     if (hipMalloc((void **)&ctx->d_info, sizeof(int)) != hipSuccess) { delete ctx; return CURFP_STATUS_ALLOC_FAILED; }
+    if (hipMalloc(&ctx->sweep_sync, SWEEP_SYNC_BYTES) != hipSuccess ||
+        hipMemset(ctx->sweep_sync, 0, SWEEP_SYNC_BYTES) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
+        (void)hipFree(ctx->d_info);
+        if (ctx->sweep_sync) (void)hipFree(ctx->sweep_sync);
+        delete ctx;
+        return CURFP_STATUS_ALLOC_FAILED;
+    }
     *handle = ctx;
     return CURFP_STATUS_SUCCESS;
 }
@@ -85,6 +94,7 @@ extern "C" curfpStatus_t curfpDestroy(curfpHandle_t handle)
     if (handle->ws) (void)hipFree(handle->ws);
     if (handle->sh) (void)hipFree(handle->sh);
     if (handle->d_info) (void)hipFree(handle->d_info);
+    if (handle->sweep_sync) (void)hipFree(handle->sweep_sync);
     delete handle;
     return CURFP_STATUS_SUCCESS;
 }
@@ -632,6 +642,18 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
                   [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
     RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
                   [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
+    if (!Prec<T>::f64 && nrhs <= 64 * SWEEP_MAX_GROUPS) {
+        /* FP32: one sweep launch per direction over all 128-row blocks */
+        const double fl = (double)g.n * (double)g.n * (double)nrhs;
+        for (int dir = 0; dir < 2; ++dir) {
+            const unsigned long long ep = ++h->sweep_epoch;
+            RFP_TRY(timed(h, CURFP_KERNEL_TRSV, fl, 0.0, [&] {
+                return launch_sweep_solve_f32((const float *)A, g, (const float *)inv, (float *)B, bv, (int)nrhs,
+                                              dir == 1, h->sweep_sync, ep, h->stream);
+            }));
+        }
+        return hipSuccess;
+    }
     Solver<T> sv{h, A, B, inv, leaf, nrhs, inv + pftrs_inv_elems(g, Prec<T>::f64)};
     const RfpView b2 = bv.sub(g.n1, 0);
     /* forward: L y = b */

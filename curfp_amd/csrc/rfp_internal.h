@@ -149,6 +149,8 @@ struct curfpContext {
     void           *sh;        /* cached BF16 shadow of the factor (BF16 math modes) */
     size_t          sh_bytes;
     int            *d_info;    /* device int: first failing minor (INT_MAX = none) */
+    void           *sweep_sync;   /* SWEEP_SYNC_BYTES of sweep-solve progress words */
+    unsigned long long sweep_epoch;
     /* kernel profiling (curfpSetProfiling / curfpGetKernelStats) */
     bool            prof;
     std::vector<RfpProfRecord> prof_pending;
@@ -241,6 +243,19 @@ hipError_t launch_trtri_tiles_f64(const double *base, RfpView v, int64_t m, doub
 int block_solve_rows();
 hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const float *inv, float *Bbase, RfpView Bv,
                                   int nrhs, bool trans, hipStream_t s);
+
+/*
+ * FP32 substitution over the whole RFP factor in one launch (trsv_block.hip):
+ * B (n x nrhs view) := L^{-1} B (trans = false) or L^{-T} B (trans = true),
+ * given the inverses of the 128-row diagonal tiles of v11 then v22 at inv.
+ * `sync` holds SWEEP_SYNC_BYTES of device words owned by the handle (zeroed
+ * once at creation); `epoch` must grow by one per launch on that handle.
+ * Supports nrhs <= 64 * SWEEP_MAX_GROUPS.
+ */
+#define SWEEP_MAX_GROUPS 64
+#define SWEEP_SYNC_BYTES 1024
+hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, float *B, RfpView bv, int nrhs,
+                                  bool trans, void *sync, unsigned long long epoch, hipStream_t s);
 
 /* RFP <-> dense row-major triangle copies (all 8 variants). */
 hipError_t launch_trttf_f32(const RfpGeom &g, bool lower, const float *A, int64_t lda, float *arf, hipStream_t s);

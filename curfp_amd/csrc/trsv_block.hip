@@ -1,18 +1,26 @@
 /*
- * trsv_block.hip — triangular solve of one diagonal block (up to 512 rows)
- * against a few right-hand sides, the leaf of spftrs' recursive block
- * substitution.
+ * trsv_block.hip — spftrs' triangular substitutions.
  *
  * The reference solves with cublasStrsm over whole RFP sub-blocks
- * (src/curfp_spftrs.cpp:258-299).  Here spftrs recurses on the RFP views
- * down to 512-row diagonal blocks (the off-diagonal updates are view GEMMs)
- * and each block is solved by this kernel: one workgroup per 16 right-hand
- * side columns keeps its 512 x 16 slice of B in LDS and walks the block's
- * 128 x 128 leaves in order — forward:  Y_q = inv(L_qq) B_q, then
- * B_r -= L_rq Y_q for the leaves r below; backward (L^T X = B) in reverse
- * with transposed blocks — each product a v_mfma_f32_16x16x4_f32 sequence
- * over a 128 x 128 operand staged in LDS.  The leaf inverses come from the
- * batched inversion kernel (trtri_tiles).
+ * (src/curfp_spftrs.cpp:258-299).  Two kernels here:
+ *
+ * sweep_solve_kernel (FP32, nrhs <= 4096): one launch per direction over the
+ * whole factor.  The factor is cut into 128-row blocks (v11's, then v22's);
+ * one workgroup per block and group of <= 64 right-hand sides accumulates
+ * the products of its row of L tiles with the already solved blocks, then
+ * applies the diagonal tile's inverse.  Workgroups publish finished blocks
+ * through a progress word and wait on it before using a block, so the
+ * substitution's serial chain runs at the speed of one tile product per
+ * block instead of one kernel launch per recursion step.
+ *
+ * block_solve_kernel (used by the recursive substitution for more right-hand
+ * sides): one workgroup per 16 right-hand-side columns keeps its 512 x 16
+ * slice of B in LDS and walks the block's 128 x 128 leaves in order —
+ * forward:  Y_q = inv(L_qq) B_q, then B_r -= L_rq Y_q for the leaves r below;
+ * backward (L^T X = B) in reverse with transposed blocks — each product a
+ * v_mfma_f32_16x16x4_f32 sequence over a 128 x 128 operand staged in LDS.
+ *
+ * The leaf inverses come from the batched inversion kernel (trtri_tiles).
  */
 #include "rfp_internal.h"
 
@@ -50,9 +58,12 @@ struct Staged {
     {
         const int c4 = (tid % 32) * 4, o0 = tid / 32;
         const bool kc = b.cs == 1;
-        /* kc: lanes run along k (c4 = k), o = i; else along i (c4 = i), o = k */
-        const int vc = kc ? b.vk : b.vi, vo = kc ? b.vi : b.vk;
-        const int64_t so = kc ? b.rs : b.cs;
+        /* kc: lanes run along k (c4 = k), o = i; else along i (c4 = i), o = k
+         * (values, not member lvalues, in the selects: no stack copy of b) */
+        const int vi = b.vi, vk = b.vk;
+        const int64_t rs = b.rs, cs = b.cs;
+        const int vc = kc ? vk : vi, vo = kc ? vi : vk;
+        const int64_t so = kc ? rs : cs;
         const float *p = b.base + b.off + c4 + (int64_t)o0 * so;
         const int64_t step = 8 * so;
         const bool full = c4 + 4 <= vc;
@@ -231,6 +242,422 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
     }
 }
 
+
+/* ------------------------------------------------------------------------ */
+/* Whole-factor sweep: one launch per substitution direction                 */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * LDS layouts of the sweep.  The MFMA k index of lane group lk (= lane / 16)
+ * in the s-th product of a 16-wide k chunk is 4 lk + s, so a lane's four
+ * operand values of a chunk are consecutive in k:
+ *   L tile, k contiguous in HBM:  As[i*SP + k]    (one 16-byte read per chunk)
+ *   L tile, i contiguous in HBM:  As[k*SP + i]    (four 4-byte reads)
+ *   Y block:                      Ys[col*SP + k]  (one 16-byte read)
+ * SP = 132 keeps the 16-byte stores in HBM order and all of these reads free
+ * of bank conflicts.  A row-major B is transposed into Ys by 4-byte stores.
+ */
+constexpr int SP = LF + 4;
+__host__ __device__ constexpr int ys_elems(int cw) { return cw * SP; }
+
+struct SweepArgs {
+    const float *L;
+    RfpView v11, v21, v22;
+    int64_t n1, n2;
+    int s1, nb;                   /* 128-row blocks in v11, in all */
+    const float *inv;             /* inverse of block b's diagonal tile at inv + b*128*128 (row-major) */
+    float *B;
+    RfpView bv;                   /* B(i, c) = B[bv.off + i*bv.rs + c*bv.cs] */
+    int nrhs, groups;
+    unsigned long long *prog;     /* [groups]: (epoch << 32) | blocks finished, in solve order */
+    unsigned *tick;               /* [0] ticket counter, [1] finished workgroups */
+    unsigned long long epoch;
+    int *err;
+};
+
+/* global first row, view-local first row and row count of block b */
+__device__ __forceinline__ void sweep_block(const SweepArgs &a, int b, int64_t &grow, int64_t &lrow, int &rows)
+{
+    if (b < a.s1) {
+        lrow = (int64_t)b * LF;
+        grow = lrow;
+        rows = (int)min<int64_t>(LF, a.n1 - lrow);
+    } else {
+        lrow = (int64_t)(b - a.s1) * LF;
+        grow = a.n1 + lrow;
+        rows = (int)min<int64_t>(LF, a.n2 - lrow);
+    }
+}
+
+/* block (r, c), r > c, of L as a view */
+__device__ __forceinline__ RfpView sweep_tile(const SweepArgs &a, int r, int c)
+{
+    int64_t gr, lr, gc, lc;
+    int mr, mc;
+    sweep_block(a, r, gr, lr, mr);
+    sweep_block(a, c, gc, lc, mc);
+    if (r < a.s1) return a.v11.sub(lr, lc);
+    if (c < a.s1) return a.v21.sub(lr, lc);
+    return a.v22.sub(lr, lc);
+}
+
+/* A thread's share of one Y block (128 x CW) moved between B and LDS as
+ * 16-byte vectors along B's contiguous direction. */
+template <int CW, bool YCM>
+struct YStage {
+    static constexpr int PER = LF * CW / 4 / 256;
+    f32x4v v[PER];
+
+    /* element e of the thread: (row, col) of the block, 4 consecutive along
+     * rows (column-major B) or along columns (row-major B) */
+    __device__ __forceinline__ static void pos(int tid, int e, int &row, int &col)
+    {
+        if (YCM) {
+            row = (tid % 32) * 4;
+            col = tid / 32 + 8 * e;
+        } else {
+            constexpr int TPR = CW / 4;   /* threads per row */
+            col = (tid % TPR) * 4;
+            row = tid / TPR + (256 / TPR) * e;
+        }
+    }
+    __device__ __forceinline__ static float *lds(float *Ys, int row, int col) { return Ys + col * SP + row; }
+    __device__ __forceinline__ void load(const SweepArgs &a, int64_t grow, int rows, int col0, int cols, int tid)
+    {
+        const int64_t sr = a.bv.rs, sc = a.bv.cs;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            int row, col;
+            pos(tid, e, row, col);
+            const float *p = a.B + a.bv.off + (grow + row) * sr + (int64_t)(col0 + col) * sc;
+            const bool full = YCM ? (row + 4 <= rows && col < cols) : (row < rows && col + 4 <= cols);
+            if (full) {
+                const f32x4u x = *(const f32x4u *)p;
+                v[e] = f32x4v{x[0], x[1], x[2], x[3]};
+            } else {
+                const int64_t st = YCM ? sr : sc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool ok = YCM ? (row + q < rows && col < cols) : (row < rows && col + q < cols);
+                    v[e][q] = ok ? p[q * st] : 0.f;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void store_lds(float *Ys, int tid) const
+    {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            int row, col;
+            pos(tid, e, row, col);
+            if (YCM) {
+                *(f32x4v *)lds(Ys, row, col) = v[e];
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) *lds(Ys, row, col + q) = v[e][q];
+            }
+        }
+    }
+    __device__ __forceinline__ void load_lds(float *Ys, int tid)
+    {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            int row, col;
+            pos(tid, e, row, col);
+            if (YCM) {
+                v[e] = *(const f32x4v *)lds(Ys, row, col);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) v[e][q] = *lds(Ys, row, col + q);
+            }
+        }
+    }
+    __device__ __forceinline__ void store(const SweepArgs &a, int64_t grow, int rows, int col0, int cols, int tid) const
+    {
+        const int64_t sr = a.bv.rs, sc = a.bv.cs;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            int row, col;
+            pos(tid, e, row, col);
+            float *p = a.B + a.bv.off + (grow + row) * sr + (int64_t)(col0 + col) * sc;
+            const bool full = YCM ? (row + 4 <= rows && col < cols) : (row < rows && col + 4 <= cols);
+            if (full) {
+                *(f32x4u *)p = f32x4u{v[e][0], v[e][1], v[e][2], v[e][3]};
+            } else {
+                const int64_t st = YCM ? sr : sc;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const bool ok = YCM ? (row + q < rows && col < cols) : (row < rows && col + q < cols);
+                    if (ok) p[q * st] = v[e][q];
+                }
+            }
+        }
+    }
+};
+
+/* The wave's two 16-row tiles of a 128-row block: w and 7 - w, so that the
+ * triangular diagonal solve gives every wave the same number of k chunks. */
+__device__ __forceinline__ int sweep_row_tile(int wave, int t) { return t == 0 ? wave : 7 - wave; }
+
+/* Operands of one 16-wide k chunk: the lane's 4 consecutive k of each of its
+ * two A rows and CT B columns. */
+template <int CT> struct ChunkOps {
+    f32x4v a[2], b[CT];
+};
+
+template <int CT, bool AKC>
+__device__ __forceinline__ void chunk_load(ChunkOps<CT> &o, const float *As, const float *Ys, int q, int wave,
+                                           int lane)
+{
+    const int li = lane & 15, k = q * 16 + 4 * (lane >> 4);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        const int i = sweep_row_tile(wave, t) * 16 + li;
+        if (AKC) {
+            o.a[t] = *(const f32x4v *)(As + i * SP + k);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; ++s) o.a[t][s] = As[(k + s) * SP + i];
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ++ct) o.b[ct] = *(const f32x4v *)(Ys + (ct * 16 + li) * SP + k);
+}
+
+/* acc[t][ct] (row tile sweep_row_tile(wave, t), columns 16 ct ..) += A (128 x
+ * 128) * Y, the next chunk's operands read while the current chunk's MFMAs
+ * run.  TRI = 1 / 2: A is lower / upper triangular by 16 x 16 tiles (the
+ * diagonal block's inverse), and the k chunks that only meet zero tiles of
+ * a row tile are skipped. */
+template <int CT, bool AKC, int TRI>
+__device__ __forceinline__ void mma_sweep(f32x4v (&acc)[2][CT], const float *As, const float *Ys, int wave, int lane)
+{
+    auto need = [&](int t, int q) {
+        const int rt = sweep_row_tile(wave, t);
+        return TRI == 0 || (TRI == 1 ? q <= rt : q >= rt);
+    };
+    ChunkOps<CT> o[2];
+    chunk_load<CT, AKC>(o[0], As, Ys, 0, wave, lane);
+#pragma unroll
+    for (int q = 0; q < LF / 16; ++q) {
+        if (q + 1 < LF / 16) chunk_load<CT, AKC>(o[(q + 1) & 1], As, Ys, q + 1, wave, lane);
+        const ChunkOps<CT> &c = o[q & 1];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            if (!need(t, q)) continue;
+#pragma unroll
+            for (int s = 0; s < 4; ++s)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+                    acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(c.a[t][s], c.b[ct][s], acc[t][ct], 0, 0, 0);
+        }
+    }
+}
+
+/* 16-byte stores of a staged L tile (or inverse) into As in the sweep layout */
+__device__ __forceinline__ void store_sweep(const Staged &st, float *As, int tid)
+{
+    const int c4 = (tid % 32) * 4, o0 = tid / 32;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) *(f32x4v *)(As + (o0 + 8 * e) * SP + c4) = st.v[e];
+}
+
+/* Bound on a workgroup's wait for an earlier block (2^20 polls of >= ~250 ns
+ * each, a quarter of a second or more, against a few milliseconds for a whole
+ * sweep), so that every wave exits even if the progress words were corrupted.
+ * After a timeout the workgroup stops waiting, reports it in *err and writes
+ * its block as NaN. */
+constexpr int SWEEP_SPIN_LIMIT = 1 << 20;
+
+/*
+ * Forward (L Y = B) or backward (L^T X = Y) substitution over the whole
+ * factor in one launch.  Workgroups take tickets in order; ticket t solves
+ * block o = t / groups in solve order (block b = o forward, nb - 1 - o
+ * backward) for the CW columns of group g = t % groups:
+ *
+ *   forward:   Y_b = inv(L_bb) (B_b - sum_{c < b} L_bc Y_c)
+ *   backward:  X_b = inv(L_bb)^T (Y_b - sum_{c > b} L_cb^T X_c)
+ *
+ * The contributions are taken in solve order, each after waiting until
+ * group g's progress word says block c is finished; then the workgroup
+ * publishes its own block.  A workgroup only ever waits for workgroups with
+ * smaller tickets, which have already started (tickets are handed out at
+ * start), so the sweep completes whatever the number of resident
+ * workgroups.  Steps 0 .. nc-1 are the contributions and step nc the
+ * diagonal solve; the operand of step j + 2 (an L tile, or the inverse) is
+ * loaded into registers while step j runs, and the next Y block (or B_b for
+ * the final step) as soon as it is known to be finished.
+ */
+template <int CW, bool TRANS, bool YCM>
+__global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
+{
+    constexpr int CT = CW / 16;
+    __shared__ __attribute__((aligned(16))) float As[LF * SP];
+    __shared__ __attribute__((aligned(16))) float Ys[ys_elems(CW)];
+    __shared__ unsigned s_ticket;
+    __shared__ int s_seen, s_fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+
+    if (tid == 0) {
+        s_ticket = __hip_atomic_fetch_add(a.tick, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_seen = 0;
+        s_fail = 0;
+    }
+    __syncthreads();
+    /* wave-uniform values in scalar registers */
+    const unsigned ticket = __builtin_amdgcn_readfirstlane(s_ticket);
+    const int o = (int)(ticket / (unsigned)a.groups), g = (int)(ticket % (unsigned)a.groups);
+    const int b = TRANS ? a.nb - 1 - o : o;
+    const int col0 = g * CW, cols = min(CW, a.nrhs - col0);
+    int64_t grow, lrow;
+    int rows;
+    sweep_block(a, b, grow, lrow, rows);
+
+    /* contribution j (solve order) comes from block c(j); step nc is the
+     * diagonal tile's inverse (transposed backward) */
+    const int nc = o;
+    auto cblk = [&](int j) { return TRANS ? a.nb - 1 - j : j; };
+    auto tile_of = [&](int j) {
+        if (j == nc) {
+            const float *W = a.inv + (int64_t)b * LF * LF;
+            return TRANS ? Blk{W, 0, 1, LF, rows, rows} : Blk{W, 0, LF, 1, rows, rows};
+        }
+        const int c = cblk(j);
+        int64_t gc, lc;
+        int mc;
+        sweep_block(a, c, gc, lc, mc);
+        if (TRANS) {
+            const RfpView v = sweep_tile(a, c, b);      /* A(i, k) = L(c-row k, b-row i) */
+            return Blk{a.L, v.off, v.cs, v.rs, rows, mc};
+        }
+        const RfpView v = sweep_tile(a, b, c);
+        return Blk{a.L, v.off, v.rs, v.cs, rows, mc};
+    };
+    /* wait until the first `need` blocks in solve order are finished */
+    auto wait_blocks = [&](int need) {
+        if (tid == 0) {
+            const unsigned long long want = (a.epoch << 32) | (unsigned long long)need;
+            unsigned long long v = 0;
+            int it = 0;
+            for (;;) {
+                v = __hip_atomic_load(a.prog + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (v >= want) break;
+                if (++it > SWEEP_SPIN_LIMIT) {
+                    __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    s_fail = 1;
+                    v = 0x7fffffffull;   /* no further waits */
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            s_seen = (int)(v & 0xffffffffu);
+        }
+        __syncthreads();
+    };
+    auto load_y = [&](YStage<CW, YCM> &y, int j) {
+        int64_t gc = grow, lc;
+        int mc = rows;
+        if (j < nc) sweep_block(a, cblk(j), gc, lc, mc);
+        y.load(a, gc, mc, col0, cols, tid);
+    };
+
+    f32x4v acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[t][ct][r] = 0.f;
+
+    Staged st[2];
+    bool kc0, kc1 = false;
+    {
+        const Blk bk = tile_of(0);
+        kc0 = bk.cs == 1;
+        st[0].load(bk, tid);
+    }
+    if (nc >= 1) {
+        const Blk bk = tile_of(1);
+        kc1 = bk.cs == 1;
+        st[1].load(bk, tid);
+    }
+    YStage<CW, YCM> yv;
+    bool yready = false;
+    int seen = 0;
+    for (int j = 0; j <= nc; ++j) {
+        const int x = j & 1;
+        if (!yready) {
+            if (j < nc && seen < j + 1) {
+                wait_blocks(j + 1);
+                seen = __builtin_amdgcn_readfirstlane(s_seen);
+            }
+            load_y(yv, j);
+        }
+        __syncthreads();   /* the previous step's MMAs are done with As and Ys */
+        const bool kc = x ? kc1 : kc0;
+        if (x) store_sweep(st[1], As, tid);
+        else   store_sweep(st[0], As, tid);
+        yv.store_lds(Ys, tid);
+        __syncthreads();
+        if (j + 2 <= nc) {
+            const Blk bk = tile_of(j + 2);
+            if (x) { kc1 = bk.cs == 1; st[1].load(bk, tid); }
+            else   { kc0 = bk.cs == 1; st[0].load(bk, tid); }
+        }
+        if (j < nc) {
+            /* B_b is always ready; Y of contribution j + 1 once it is known finished */
+            yready = j + 1 == nc || seen >= j + 2;
+            if (yready) load_y(yv, j + 1);
+            if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
+            else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
+        } else {
+            /* R = B_b - acc in place in Ys, then acc = inv(L_bb) R (transposed backward) */
+#pragma unroll
+            for (int t = 0; t < 2; ++t)
+#pragma unroll
+                for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
+                        *YStage<CW, YCM>::lds(Ys, row, col) -= acc[t][ct][r];
+                        acc[t][ct][r] = 0.f;
+                    }
+            __syncthreads();
+            /* inv(L_bb) is lower triangular, its transpose upper */
+            if (TRANS) mma_sweep<CT, false, 2>(acc, As, Ys, wave, lane);
+            else       mma_sweep<CT, true, 1>(acc, As, Ys, wave, lane);
+        }
+    }
+    __syncthreads();
+    const float nan = __builtin_nanf("");
+    const bool fail = s_fail != 0;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
+                *YStage<CW, YCM>::lds(Ys, row, col) = fail ? nan : acc[t][ct][r];
+            }
+    __syncthreads();
+    yv.load_lds(Ys, tid);
+    yv.store(a, grow, rows, col0, cols, tid);
+    /* publish: every wave's stores are written back before the progress word moves */
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (tid == 0) {
+        __hip_atomic_store(a.prog + g, (a.epoch << 32) | (unsigned long long)(o + 1), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        /* the last workgroup to finish re-arms the ticket counter for the next launch */
+        if (__hip_atomic_fetch_add(a.tick + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+            __hip_atomic_store(a.tick, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(a.tick + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 }  // namespace
 
 int block_solve_rows() { return MAXQ * LF; }
@@ -246,5 +673,41 @@ hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const f
     else
         hipLaunchKernelGGL((block_solve_kernel<false>), dim3(grid), dim3(256), 0, s, Lbase, Lv, m, inv, Bbase, Bv,
                            nrhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, float *B, RfpView bv, int nrhs,
+                                  bool trans, void *sync, unsigned long long epoch, hipStream_t s)
+{
+    if (g.n <= 0 || nrhs <= 0) return hipSuccess;
+    const int cw = nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : 64;
+    const int groups = (nrhs + cw - 1) / cw;
+    if (groups > SWEEP_MAX_GROUPS) return hipErrorInvalidValue;
+    SweepArgs a;
+    a.L = L;
+    a.v11 = g.v11; a.v21 = g.v21; a.v22 = g.v22;
+    a.n1 = g.n1; a.n2 = g.n2;
+    a.s1 = (int)((g.n1 + LF - 1) / LF);
+    a.nb = a.s1 + (int)((g.n2 + LF - 1) / LF);
+    a.inv = inv;
+    a.B = B; a.bv = bv;
+    a.nrhs = nrhs; a.groups = groups;
+    a.prog = (unsigned long long *)sync;
+    a.tick = (unsigned *)((unsigned long long *)sync + SWEEP_MAX_GROUPS);
+    a.err = (int *)(a.tick + 2);
+    a.epoch = epoch;
+    const unsigned grid = (unsigned)a.nb * (unsigned)groups;
+    const bool ycm = bv.rs == 1;
+#define SWEEP_LAUNCH(CW_, T_, C_) hipLaunchKernelGGL((sweep_solve_kernel<CW_, T_, C_>), dim3(grid), dim3(256), 0, s, a)
+#define SWEEP_LAUNCH_CW(CW_)                                                \
+    do {                                                                    \
+        if (trans) { if (ycm) SWEEP_LAUNCH(CW_, true, true);  else SWEEP_LAUNCH(CW_, true, false); }   \
+        else       { if (ycm) SWEEP_LAUNCH(CW_, false, true); else SWEEP_LAUNCH(CW_, false, false); }  \
+    } while (0)
+    if (cw == 16) SWEEP_LAUNCH_CW(16);
+    else if (cw == 32) SWEEP_LAUNCH_CW(32);
+    else SWEEP_LAUNCH_CW(64);
+#undef SWEEP_LAUNCH_CW
+#undef SWEEP_LAUNCH
     return hipGetLastError();
 }

@@ -274,6 +274,45 @@ def test_spftrs_1d_and_column_major_abi(curfp):
     assert np.all(Bt.cpu().numpy()[:, n:] == 0)
 
 
+@pytest.mark.parametrize("n,t,u", [(777, "T", "U"), (1025, "N", "L"), (1154, "N", "U"), (300, "T", "L")])
+def test_spftrs_sweep_widths(curfp, n, t, u):
+    """FP32 spftrs runs one substitution launch per direction over 128-row
+    blocks, with 16/32/64 right-hand sides per workgroup and several column
+    groups beyond 64.  Every width and group count, partial column groups,
+    both B layouts and ragged last blocks (n1, n2 not multiples of 128) must
+    solve to the FP32 residual bound; repeated calls (the launches' progress
+    words are reused) must give bit-identical results; more than 4096
+    right-hand sides take the recursive block substitution."""
+    k = 96
+    A, _, S = _spd_problem(n, k, 1, seed=11)
+    C = torch.empty(n * (n + 1) // 2, dtype=torch.float32, device="cuda")
+    curfp.ssfrk(dev(A), C, transr=t, uplo=u)
+    curfp.add_to_diagonal(C, 1.0, transr=t, uplo=u)
+    assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    rng = np.random.default_rng(n)
+    widths = [1, 16, 17, 33, 64, 65, 200] + ([4100] if n == 300 else [])
+    for nrhs in widths:
+        B = rng.standard_normal((n, nrhs)).astype(np.float32)
+        X = dev(B)
+        curfp.spftrs(C, X, transr=t, uplo=u)
+        res = O.dense_relative_residual(S, X.cpu().numpy(), B)
+        assert res <= 1e-6, (nrhs, res)
+        X2 = dev(B)
+        curfp.spftrs(C, X2, transr=t, uplo=u)
+        assert torch.equal(X, X2), nrhs
+        if nrhs in (17, 65):
+            # column-major B with ldb > n through the C ABI
+            ldb = n + 5
+            Bc = np.zeros((nrhs, ldb), np.float32)
+            Bc[:, :n] = B.T
+            Bt = dev(Bc)
+            curfp.spftrs_raw(curfp._get_handle(), curfp.OP_T if t == "T" else curfp.OP_N,
+                             curfp.FILL_UPPER if u == "U" else curfp.FILL_LOWER, n, nrhs, C, Bt, ldb)
+            Xc = Bt.cpu().numpy()[:, :n].T
+            assert O.dense_relative_residual(S, Xc, B) <= 1e-6, nrhs
+            assert np.all(Bt.cpu().numpy()[:, n:] == 0)
+
+
 def test_stream_binding(curfp):
     """Operations follow the stream bound with set_stream (test_spftrs.py:169-188)."""
     n, k = 513, 64
