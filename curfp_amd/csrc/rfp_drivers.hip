@@ -432,6 +432,16 @@ struct Factor {
     hipError_t trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return hipSuccess;
+        if (!Prec<T>::f64 && m <= trsm_panel_max()) {
+            /* FP32: the last levels of the recursion in one launch, each
+             * workgroup substituting a slab of rows through all leaves */
+            const double fl = (double)mb * (double)m * (double)m;
+            const double by = (double)mb * m * 4.0 * 2.0 + (double)m * m * 4.0 + (sh_hi ? (double)mb * m * 2.0 : 0.0);
+            return timed(h, CURFP_KERNEL_GEMM_F32, fl, by, [&] {
+                return launch_trsm_panel_f32((float *)a, B, mb, L, m, (const float *)(inv + slot * leaf * leaf),
+                                             sh_hi, sh_lo, Bs, h->stream);
+            });
+        }
         if (m <= leaf)
             return gemm(B, a, B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, mb, m, m, 1.0, 0.0,
                         false, 1, RFP_MATH_F32, &Bs);

@@ -245,6 +245,17 @@ hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const f
                                   int nrhs, bool trans, hipStream_t s);
 
 /*
+ * FP32 X := X * L^{-T} (trsm_panel.hip) for X an mb x m view of the RFP array
+ * and L an m x m lower-triangular view with m <= trsm_panel_max() (at most
+ * four 128-row leaves, whose inverses are at inv + q*128*128, row-major).
+ * When sh_hi is set the result is also rounded into the BF16 shadow view S
+ * (and its low part into sh_lo).
+ */
+int trsm_panel_max();
+hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int64_t m, const float *inv,
+                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, hipStream_t s);
+
+/*
  * FP32 substitution over the whole RFP factor in one launch (trsv_block.hip):
  * B (n x nrhs view) := L^{-1} B (trans = false) or L^{-T} B (trans = true),
  * given the inverses of the 128-row diagonal tiles of v11 then v22 at inv.

@@ -23,74 +23,17 @@
  * The leaf inverses come from the batched inversion kernel (trtri_tiles).
  */
 #include "rfp_internal.h"
+#include "tile_stage.h"
 
 namespace {
 
-typedef float f32x4v __attribute__((ext_vector_type(4)));
-
-constexpr int LF = 128;        /* leaf */
+constexpr int LF = TILE_LF;    /* leaf */
 constexpr int MAXQ = 4;        /* leaves per block: 512 rows */
 constexpr int CW = 16;         /* right-hand-side columns per workgroup */
 constexpr int AP = LF + 4;     /* LDS pitch of a staged operand held as As[i][k] */
 constexpr int ATP = LF + 16;   /* LDS pitch of a staged operand held as As[k][i] */
 constexpr int MAXSTEPS = MAXQ + MAXQ * (MAXQ - 1) / 2;
 constexpr int BP = CW + 1;     /* LDS pitch of the B slice */
-
-/* The 128 x 128 operand A(i, k) = base[off + i*rs + k*cs] of one step (rows /
- * columns at or beyond vi / vk read as zero). */
-struct Blk {
-    const float *base;
-    int64_t off, rs, cs;
-    int vi, vk;
-};
-
-/* 16-byte vector that only assumes element alignment (RFP leading dimensions
- * such as n + 1 leave rows unaligned) */
-typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
-
-/* A thread's 64 elements of a staged operand as 16 vectors of 4 consecutive
- * elements along the contiguous direction of the source (eight rows /
- * columns apart). */
-struct Staged {
-    f32x4v v[16];
-
-    __device__ __forceinline__ void load(const Blk &b, int tid)
-    {
-        const int c4 = (tid % 32) * 4, o0 = tid / 32;
-        const bool kc = b.cs == 1;
-        /* kc: lanes run along k (c4 = k), o = i; else along i (c4 = i), o = k
-         * (values, not member lvalues, in the selects: no stack copy of b) */
-        const int vi = b.vi, vk = b.vk;
-        const int64_t rs = b.rs, cs = b.cs;
-        const int vc = kc ? vk : vi, vo = kc ? vi : vk;
-        const int64_t so = kc ? rs : cs;
-        const float *p = b.base + b.off + c4 + (int64_t)o0 * so;
-        const int64_t step = 8 * so;
-        const bool full = c4 + 4 <= vc;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) {
-            const int o = o0 + 8 * e;
-            const float *q = p + e * step;
-            if (o < vo && full) {
-                const f32x4u x = *(const f32x4u *)q;
-                v[e] = f32x4v{x[0], x[1], x[2], x[3]};
-            } else {
-#pragma unroll
-                for (int c = 0; c < 4; ++c) v[e][c] = (o < vo && c4 + c < vc) ? q[c] : 0.f;
-            }
-        }
-    }
-    /* into LDS along the source's contiguous direction: As[i*AP + k] when k
-     * is contiguous (kc), else As[k*ATP + i] (both pitches keep the 16-byte
-     * stores aligned and the MFMA operand reads free of bank conflicts) */
-    __device__ __forceinline__ void store(float *As, bool kc, int tid) const
-    {
-        const int c4 = (tid % 32) * 4, o0 = tid / 32;
-        const int pitch = kc ? AP : ATP;
-#pragma unroll
-        for (int e = 0; e < 16; ++e) *(f32x4v *)(As + (o0 + 8 * e) * pitch + c4) = v[e];
-    }
-};
 
 /*
  * acc (rows wave*32 .. +32, all 16 columns, as two 16 x 16 tiles) =
@@ -218,7 +161,7 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
             const int x = j & 1;
             const bool kc = bk[x].cs == 1;
             const int cq = qs[x], cr = rs[x];
-            st[x].store(As, kc, tid);
+            st[x].store(As, kc ? AP : ATP, tid);
             __syncthreads();
             if (j + 2 < nsteps) {
                 step_of(j + 2, qs[x], rs[x]);
@@ -454,14 +397,6 @@ __device__ __forceinline__ void mma_sweep(f32x4v (&acc)[2][CT], const float *As,
     }
 }
 
-/* 16-byte stores of a staged L tile (or inverse) into As in the sweep layout */
-__device__ __forceinline__ void store_sweep(const Staged &st, float *As, int tid)
-{
-    const int c4 = (tid % 32) * 4, o0 = tid / 32;
-#pragma unroll
-    for (int e = 0; e < 16; ++e) *(f32x4v *)(As + (o0 + 8 * e) * SP + c4) = st.v[e];
-}
-
 /* Bound on a workgroup's wait for an earlier block (2^20 polls of >= ~250 ns
  * each, a quarter of a second or more, against a few milliseconds for a whole
  * sweep), so that every wave exits even if the progress words were corrupted.
@@ -594,21 +529,26 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             }
             load_y(yv, j);
         }
-        __syncthreads();   /* the previous step's MMAs are done with As and Ys */
+        lds_barrier();     /* the previous step's MMAs are done with As and Ys */
         const bool kc = x ? kc1 : kc0;
-        if (x) store_sweep(st[1], As, tid);
-        else   store_sweep(st[0], As, tid);
+        if (x) st[1].store(As, SP, tid);
+        else   st[0].store(As, SP, tid);
         yv.store_lds(Ys, tid);
-        __syncthreads();
+        lds_barrier();
+        /* next Y block (or B_b for the final step) before the L tile two steps
+         * ahead: the loads complete in issue order, so the next step waits
+         * for its Y block without waiting for that tile */
+        if (j < nc) {
+            /* B_b is always ready; Y of contribution j + 1 once it is known finished */
+            yready = j + 1 == nc || seen >= j + 2;
+            if (yready) load_y(yv, j + 1);
+        }
         if (j + 2 <= nc) {
             const Blk bk = tile_of(j + 2);
             if (x) { kc1 = bk.cs == 1; st[1].load(bk, tid); }
             else   { kc0 = bk.cs == 1; st[0].load(bk, tid); }
         }
         if (j < nc) {
-            /* B_b is always ready; Y of contribution j + 1 once it is known finished */
-            yready = j + 1 == nc || seen >= j + 2;
-            if (yready) load_y(yv, j + 1);
             if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
             else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
         } else {
@@ -623,13 +563,13 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
                         *YStage<CW, YCM>::lds(Ys, row, col) -= acc[t][ct][r];
                         acc[t][ct][r] = 0.f;
                     }
-            __syncthreads();
+            lds_barrier();
             /* inv(L_bb) is lower triangular, its transpose upper */
             if (TRANS) mma_sweep<CT, false, 2>(acc, As, Ys, wave, lane);
             else       mma_sweep<CT, true, 1>(acc, As, Ys, wave, lane);
         }
     }
-    __syncthreads();
+    lds_barrier();
     const float nan = __builtin_nanf("");
     const bool fail = s_fail != 0;
 #pragma unroll
@@ -641,7 +581,7 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
                 const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
                 *YStage<CW, YCM>::lds(Ys, row, col) = fail ? nan : acc[t][ct][r];
             }
-    __syncthreads();
+    lds_barrier();
     yv.load_lds(Ys, tid);
     yv.store(a, grow, rows, col0, cols, tid);
     /* publish: every wave's stores are written back before the progress word moves */
