@@ -115,7 +115,8 @@ KERNEL_INFO = {
                   "v_mfma_f32_32x32x16_bf16, FP32 accumulate, direct global->LDS loads)", "bf16"),
     "gemm_f64": ("view_gemm_kernel<double> (FP64 MFMA v_mfma_f64_16x16x4f64)", "fp64"),
     "potrf_tile": ("potrf_leaf_kernel (128-row Cholesky leaf + its inverse, FP32 MFMA)", "fp32"),
-    "trsv": ("block_solve_kernel (512-row block substitution, FP32 MFMA)", "fp32"),
+    "trsv": ("sweep_solve_kernel (whole-factor substitution over 128-row blocks, FP32 MFMA)", "fp32"),
+    "trsm": ("trsm_panel_kernel (spftrf X := X L^-T over <= 4 leaves per launch, FP32 MFMA)", "fp32"),
 }
 
 

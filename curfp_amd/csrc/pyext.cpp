@@ -103,6 +103,7 @@ PYBIND11_MODULE(_curfp_C, m)
     kc["diag"] = (int)CURFP_KERNEL_DIAG;
     kc["other"] = (int)CURFP_KERNEL_OTHER;
     kc["trsv"] = (int)CURFP_KERNEL_TRSV;
+    kc["trsm"] = (int)CURFP_KERNEL_TRSM;
     m.attr("KERNEL_CLASSES") = kc;
 
     /* ---- FP32 ---- */

@@ -437,7 +437,7 @@ struct Factor {
              * workgroup substituting a slab of rows through all leaves */
             const double fl = (double)mb * (double)m * (double)m;
             const double by = (double)mb * m * 4.0 * 2.0 + (double)m * m * 4.0 + (sh_hi ? (double)mb * m * 2.0 : 0.0);
-            return timed(h, CURFP_KERNEL_GEMM_F32, fl, by, [&] {
+            return timed(h, CURFP_KERNEL_TRSM, fl, by, [&] {
                 return launch_trsm_panel_f32((float *)a, B, mb, L, m, (const float *)(inv + slot * leaf * leaf),
                                              sh_hi, sh_lo, Bs, h->stream);
             });
@@ -636,9 +636,12 @@ static size_t pftrs_inv_elems(const RfpGeom &g, bool f64)
     return (size_t)(s1 + s2) * leaf * leaf;
 }
 
+/* leaf inverses, split-K partials, and (FP32) the sweep's four products per
+ * leaf */
 static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
 {
-    return (pftrs_inv_elems(g, f64) + (size_t)(8 << 20)) * (f64 ? sizeof(double) : sizeof(float));
+    const size_t inv = pftrs_inv_elems(g, f64);
+    return ((f64 ? inv : 5 * inv) + (size_t)(8 << 20)) * (f64 ? sizeof(double) : sizeof(float));
 }
 
 /* Forward + backward substitution on B (view bv, n x nrhs) with the RFP
@@ -654,12 +657,17 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
                   [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
     if (!Prec<T>::f64 && nrhs <= 64 * SWEEP_MAX_GROUPS) {
         /* FP32: one sweep launch per direction over all 128-row blocks */
+        float *pm = (float *)(inv + pftrs_inv_elems(g, false)) + ((size_t)8 << 20);
+        const double nb = (double)((g.n1 + leaf - 1) / leaf + (g.n2 + leaf - 1) / leaf);
+        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, 4.0 * nb * 2.0 * leaf * leaf * leaf, 0.0, [&] {
+            return launch_sweep_pmat_f32((const float *)A, g, (const float *)inv, pm, h->stream);
+        }));
         const double fl = (double)g.n * (double)g.n * (double)nrhs;
         for (int dir = 0; dir < 2; ++dir) {
             const unsigned long long ep = ++h->sweep_epoch;
             RFP_TRY(timed(h, CURFP_KERNEL_TRSV, fl, 0.0, [&] {
-                return launch_sweep_solve_f32((const float *)A, g, (const float *)inv, (float *)B, bv, (int)nrhs,
-                                              dir == 1, h->sweep_sync, ep, h->stream);
+                return launch_sweep_solve_f32((const float *)A, g, (const float *)inv, pm, (float *)B, bv,
+                                              (int)nrhs, dir == 1, h->sweep_sync, ep, h->stream);
             }));
         }
         return hipSuccess;

@@ -258,15 +258,20 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
 /*
  * FP32 substitution over the whole RFP factor in one launch (trsv_block.hip):
  * B (n x nrhs view) := L^{-1} B (trans = false) or L^{-T} B (trans = true),
- * given the inverses of the 128-row diagonal tiles of v11 then v22 at inv.
+ * given the inverses of the 128-row diagonal tiles of v11 then v22 at inv and
+ * the products written by launch_sweep_pmat_f32 at pm.
  * `sync` holds SWEEP_SYNC_BYTES of device words owned by the handle (zeroed
  * once at creation); `epoch` must grow by one per launch on that handle.
  * Supports nrhs <= 64 * SWEEP_MAX_GROUPS.
  */
 #define SWEEP_MAX_GROUPS 64
 #define SWEEP_SYNC_BYTES 1024
-hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, float *B, RfpView bv, int nrhs,
-                                  bool trans, void *sync, unsigned long long epoch, hipStream_t s);
+hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
+                                  RfpView bv, int nrhs, bool trans, void *sync, unsigned long long epoch,
+                                  hipStream_t s);
+/* The sweep's products with the diagonal inverses, 4 * (blocks) 128 x 128
+ * matrices at pm (written by this launch, read by both sweeps). */
+hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *inv, float *pm, hipStream_t s);
 
 /* RFP <-> dense row-major triangle copies (all 8 variants). */
 hipError_t launch_trttf_f32(const RfpGeom &g, bool lower, const float *A, int64_t lda, float *arf, hipStream_t s);

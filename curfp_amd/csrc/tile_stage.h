@@ -41,6 +41,10 @@ struct Blk {
 struct Staged {
     f32x4v v[16];
 
+    /* WHOLE: a straight-line path for whole tiles (fewer registers in
+     * kernels that stage one tile at a time; kernels that unroll many
+     * staging sites keep the single masked path) */
+    template <bool WHOLE = true>
     __device__ __forceinline__ void load(const Blk &b, int tid)
     {
         const int c4 = (tid % 32) * 4, o0 = tid / 32;
@@ -53,7 +57,7 @@ struct Staged {
         const int64_t so = kc ? rs : cs;
         const float *p = b.base + b.off + c4 + (int64_t)o0 * so;
         const int64_t step = 8 * so;
-        if (vi == TILE_LF && vk == TILE_LF) {
+        if (WHOLE && vi == TILE_LF && vk == TILE_LF) {
             /* whole tile (the common case): straight-line loads */
 #pragma unroll
             for (int e = 0; e < 16; ++e) {

@@ -153,7 +153,7 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
         if (j < nsteps) {
             step_of(j, qs[j], rs[j]);
             bk[j] = blk_of(qs[j], rs[j]);
-            st[j].load(bk[j], tid);
+            st[j].template load<false>(bk[j], tid);
         }
 #pragma unroll
     for (int j = 0; j < MAXSTEPS; ++j) {
@@ -166,7 +166,7 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
             if (j + 2 < nsteps) {
                 step_of(j + 2, qs[x], rs[x]);
                 bk[x] = blk_of(qs[x], rs[x]);
-                st[x].load(bk[x], tid);
+                st[x].template load<false>(bk[x], tid);
             }
             f32x4v acc[2];
             mma_leaf(acc, As, kc, Bs, cq * LF, wave, lane);
@@ -209,6 +209,8 @@ struct SweepArgs {
     int64_t n1, n2;
     int s1, nb;                   /* 128-row blocks in v11, in all */
     const float *inv;             /* inverse of block b's diagonal tile at inv + b*128*128 (row-major) */
+    const float *pm;              /* P_b,l = inv(L_bb) L_b,b-l and inv(L_bb)^T L_b+l,b^T (l = 1, 2)
+                                   * at pm + ((2*dir + l - 1)*nb + b)*128*128 (row-major) */
     float *B;
     RfpView bv;                   /* B(i, c) = B[bv.off + i*bv.rs + c*bv.cs] */
     int nrhs, groups;
@@ -397,6 +399,76 @@ __device__ __forceinline__ void mma_sweep(f32x4v (&acc)[2][CT], const float *As,
     }
 }
 
+/*
+ * The products P_b,l that take the diagonal inverse off the sweep's serial
+ * chain (one workgroup per block b, direction and l = 1, 2):
+ *   forward:   P_b,l = inv(L_bb) L_b,b-l,       Y_b = inv(L_bb) R_b - P_b,2 Y_b-2 - P_b,1 Y_b-1
+ *   backward:  P_b,l = inv(L_bb)^T L_b+l,b^T,   X_b = inv(L_bb)^T R_b - P_b,2 X_b+2 - P_b,1 X_b+1
+ * with R_b the right-hand side less every other contribution.  They depend
+ * only on the factor (128^3 products, once per solve).
+ */
+__global__ __launch_bounds__(256) void sweep_pmat_kernel(SweepArgs a)
+{
+    __shared__ __attribute__((aligned(16))) float Ia[LF * SP];
+    __shared__ __attribute__((aligned(16))) float Lt[LF * SP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x, dir = blockIdx.y >> 1, lag = 1 + (blockIdx.y & 1);
+    const int c = dir == 0 ? b - lag : b + lag;
+    if (c < 0 || c >= a.nb) return;
+    int64_t gb, lb, gc, lc;
+    int mb, mc;
+    sweep_block(a, b, gb, lb, mb);
+    sweep_block(a, c, gc, lc, mc);
+    {
+        Staged st;
+        st.load(Blk{a.inv + (int64_t)b * LF * LF, 0, LF, 1, mb, mb}, tid);
+        st.store(Ia, SP, tid);                                   /* Ia[i*SP + k] = inv(i, k) */
+    }
+    /* the off-diagonal tile in its natural orientation: L(r, q) at Lt[r*lr + q*lq] */
+    const RfpView v = dir == 0 ? sweep_tile(a, b, c) : sweep_tile(a, c, b);
+    const bool kc = v.cs == 1;
+    {
+        Staged st;
+        st.load(Blk{a.L, v.off, v.rs, v.cs, dir == 0 ? mb : mc, dir == 0 ? mc : mb}, tid);
+        st.store(Lt, SP, tid);
+    }
+    const int lr = kc ? SP : 1, lq = kc ? 1 : SP;
+    __syncthreads();
+    /* P(i, j) = sum_k A(i, k) B(k, j): A = inv or inv^T, B(k, j) = L_b,c(k, j) or L_c,b(j, k) */
+    const int ai = dir == 0 ? SP : 1, ak = dir == 0 ? 1 : SP;
+    const int bk = dir == 0 ? lr : lq, bj = dir == 0 ? lq : lr;
+    const int li = lane & 15, lk = lane >> 4;
+    f32x4v acc[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[t][ct][r] = 0.f;
+    for (int k0 = 0; k0 < LF; k0 += 4) {
+        const int k = k0 + lk;
+        float av[2], bv[8];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) av[t] = Ia[(wave * 32 + t * 16 + li) * ai + k * ak];
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct) bv[ct] = Lt[k * bk + (ct * 16 + li) * bj];
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[t], bv[ct], acc[t][ct], 0, 0, 0);
+    }
+    float *P = (float *)a.pm + ((int64_t)(2 * dir + lag - 1) * a.nb + b) * LF * LF;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < 8; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = wave * 32 + t * 16 + 4 * lk + r, j = ct * 16 + li;
+                P[i * LF + j] = (i < mb && j < mc) ? acc[t][ct][r] : 0.f;
+            }
+}
+
 /* Bound on a workgroup's wait for an earlier block (2^20 polls of >= ~250 ns
  * each, a quarter of a second or more, against a few milliseconds for a whole
  * sweep), so that every wave exits even if the progress words were corrupted.
@@ -418,10 +490,14 @@ constexpr int SWEEP_SPIN_LIMIT = 1 << 20;
  * publishes its own block.  A workgroup only ever waits for workgroups with
  * smaller tickets, which have already started (tickets are handed out at
  * start), so the sweep completes whatever the number of resident
- * workgroups.  Steps 0 .. nc-1 are the contributions and step nc the
- * diagonal solve; the operand of step j + 2 (an L tile, or the inverse) is
- * loaded into registers while step j runs, and the next Y block (or B_b for
- * the final step) as soon as it is known to be finished.
+ * workgroups.  For block o in solve order the steps are: the
+ * contributions of blocks 0 .. o-3 (solve order), then Z = inv * (B_b - sum)
+ * (kept in LDS), then acc = P_b,2 Y_c(o-2) + P_b,1 Y_c(o-1) and Y_b = Z - acc
+ * (fewer look-back steps for o < 2), so that only one tile product separates
+ * the previous block's publication from this block's, and the block before
+ * that leaves a whole step of slack for the rest.  The operand of step j + 2 (an L tile, the inverse or P_b) is
+ * loaded into registers while step j runs, and the next step's Y block (or
+ * B_b) as soon as it is known to be finished.
  */
 template <int CW, bool TRANS, bool YCM>
 __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
@@ -429,6 +505,7 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     constexpr int CT = CW / 16;
     __shared__ __attribute__((aligned(16))) float As[LF * SP];
     __shared__ __attribute__((aligned(16))) float Ys[ys_elems(CW)];
+    __shared__ __attribute__((aligned(16))) float Zs[ys_elems(CW)];
     __shared__ unsigned s_ticket;
     __shared__ int s_seen, s_fail;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -448,19 +525,27 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     int rows;
     sweep_block(a, b, grow, lrow, rows);
 
-    /* contribution j (solve order) comes from block c(j); step nc is the
-     * diagonal tile's inverse (transposed backward) */
-    const int nc = o;
+    /* steps: j < zs the contribution of block c(j) (solve order), j == zs the
+     * inverse applied to B_b less those contributions, j > zs the product of
+     * P_b,l with block c(j - 1), l = o + 1 - j */
+    const int npl = o < 2 ? o : 2;
+    const int zs = o - npl;
+    const int ns = o + 1;
     auto cblk = [&](int j) { return TRANS ? a.nb - 1 - j : j; };
     auto tile_of = [&](int j) {
-        if (j == nc) {
+        if (j == zs) {
             const float *W = a.inv + (int64_t)b * LF * LF;
             return TRANS ? Blk{W, 0, 1, LF, rows, rows} : Blk{W, 0, LF, 1, rows, rows};
         }
-        const int c = cblk(j);
+        const int c = cblk(j < zs ? j : j - 1);
         int64_t gc, lc;
         int mc;
         sweep_block(a, c, gc, lc, mc);
+        if (j > zs) {
+            const int l = o + 1 - j;
+            const float *P = a.pm + ((int64_t)(2 * (TRANS ? 1 : 0) + l - 1) * a.nb + b) * LF * LF;
+            return Blk{P, 0, LF, 1, rows, mc};
+        }
         if (TRANS) {
             const RfpView v = sweep_tile(a, c, b);      /* A(i, k) = L(c-row k, b-row i) */
             return Blk{a.L, v.off, v.cs, v.rs, rows, mc};
@@ -468,6 +553,8 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
         const RfpView v = sweep_tile(a, b, c);
         return Blk{a.L, v.off, v.rs, v.cs, rows, mc};
     };
+    /* blocks in solve order that must be finished before step j's Y block is read */
+    auto need_of = [&](int j) { return j < zs ? j + 1 : (j == zs ? 0 : j); };
     /* wait until the first `need` blocks in solve order are finished */
     auto wait_blocks = [&](int need) {
         if (tid == 0) {
@@ -493,7 +580,7 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     auto load_y = [&](YStage<CW, YCM> &y, int j) {
         int64_t gc = grow, lc;
         int mc = rows;
-        if (j < nc) sweep_block(a, cblk(j), gc, lc, mc);
+        if (j != zs) sweep_block(a, cblk(j < zs ? j : j - 1), gc, lc, mc);
         y.load(a, gc, mc, col0, cols, tid);
     };
 
@@ -512,7 +599,7 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
         kc0 = bk.cs == 1;
         st[0].load(bk, tid);
     }
-    if (nc >= 1) {
+    if (ns > 1) {
         const Blk bk = tile_of(1);
         kc1 = bk.cs == 1;
         st[1].load(bk, tid);
@@ -520,11 +607,12 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     YStage<CW, YCM> yv;
     bool yready = false;
     int seen = 0;
-    for (int j = 0; j <= nc; ++j) {
+    for (int j = 0; j < ns; ++j) {
         const int x = j & 1;
         if (!yready) {
-            if (j < nc && seen < j + 1) {
-                wait_blocks(j + 1);
+            const int need = need_of(j);
+            if (seen < need) {
+                wait_blocks(need);
                 seen = __builtin_amdgcn_readfirstlane(s_seen);
             }
             load_y(yv, j);
@@ -535,23 +623,22 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
         else   st[0].store(As, SP, tid);
         yv.store_lds(Ys, tid);
         lds_barrier();
-        /* next Y block (or B_b for the final step) before the L tile two steps
-         * ahead: the loads complete in issue order, so the next step waits
-         * for its Y block without waiting for that tile */
-        if (j < nc) {
-            /* B_b is always ready; Y of contribution j + 1 once it is known finished */
-            yready = j + 1 == nc || seen >= j + 2;
+        /* next step's Y block before the operand two steps ahead: the loads
+         * complete in issue order, so the next step waits for its Y block
+         * without waiting for that operand */
+        if (j + 1 < ns) {
+            yready = seen >= need_of(j + 1);
             if (yready) load_y(yv, j + 1);
         }
-        if (j + 2 <= nc) {
+        if (j + 2 < ns) {
             const Blk bk = tile_of(j + 2);
             if (x) { kc1 = bk.cs == 1; st[1].load(bk, tid); }
             else   { kc0 = bk.cs == 1; st[0].load(bk, tid); }
         }
-        if (j < nc) {
+        if (j < zs) {
             if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
             else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
-        } else {
+        } else if (j == zs) {
             /* R = B_b - acc in place in Ys, then acc = inv(L_bb) R (transposed backward) */
 #pragma unroll
             for (int t = 0; t < 2; ++t)
@@ -567,6 +654,35 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             /* inv(L_bb) is lower triangular, its transpose upper */
             if (TRANS) mma_sweep<CT, false, 2>(acc, As, Ys, wave, lane);
             else       mma_sweep<CT, true, 1>(acc, As, Ys, wave, lane);
+            if (o > 0) {
+                /* Z = acc waits in LDS for the last block */
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r;
+                            const int col = ct * 16 + (lane & 15);
+                            *YStage<CW, YCM>::lds(Zs, row, col) = acc[t][ct][r];
+                            acc[t][ct][r] = 0.f;
+                        }
+            }
+        } else {
+            /* acc += P_b,l Y_c; after the last, the block's result is Z - acc */
+            mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
+            if (j == o) {
+#pragma unroll
+                for (int t = 0; t < 2; ++t)
+#pragma unroll
+                    for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r;
+                            const int col = ct * 16 + (lane & 15);
+                            acc[t][ct][r] = *YStage<CW, YCM>::lds(Zs, row, col) - acc[t][ct][r];
+                        }
+            }
         }
     }
     lds_barrier();
@@ -616,20 +732,36 @@ hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const f
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, float *B, RfpView bv, int nrhs,
-                                  bool trans, void *sync, unsigned long long epoch, hipStream_t s)
+static SweepArgs sweep_args(const float *L, const RfpGeom &g, const float *inv, const float *pm)
 {
-    if (g.n <= 0 || nrhs <= 0) return hipSuccess;
-    const int cw = nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : 64;
-    const int groups = (nrhs + cw - 1) / cw;
-    if (groups > SWEEP_MAX_GROUPS) return hipErrorInvalidValue;
-    SweepArgs a;
+    SweepArgs a{};
     a.L = L;
     a.v11 = g.v11; a.v21 = g.v21; a.v22 = g.v22;
     a.n1 = g.n1; a.n2 = g.n2;
     a.s1 = (int)((g.n1 + LF - 1) / LF);
     a.nb = a.s1 + (int)((g.n2 + LF - 1) / LF);
     a.inv = inv;
+    a.pm = pm;
+    return a;
+}
+
+hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *inv, float *pm, hipStream_t s)
+{
+    if (g.n <= 0) return hipSuccess;
+    const SweepArgs a = sweep_args(L, g, inv, pm);
+    hipLaunchKernelGGL(sweep_pmat_kernel, dim3((unsigned)a.nb, 4), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
+                                  RfpView bv, int nrhs, bool trans, void *sync, unsigned long long epoch,
+                                  hipStream_t s)
+{
+    if (g.n <= 0 || nrhs <= 0) return hipSuccess;
+    const int cw = nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : 64;
+    const int groups = (nrhs + cw - 1) / cw;
+    if (groups > SWEEP_MAX_GROUPS) return hipErrorInvalidValue;
+    SweepArgs a = sweep_args(L, g, inv, pm);
     a.B = B; a.bv = bv;
     a.nrhs = nrhs; a.groups = groups;
     a.prog = (unsigned long long *)sync;

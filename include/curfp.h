@@ -179,8 +179,9 @@ typedef enum {
     CURFP_KERNEL_PACK       = 5,   /* strttf / stfttr                     */
     CURFP_KERNEL_DIAG       = 6,   /* add_to_diagonal                     */
     CURFP_KERNEL_OTHER      = 7,   /* fills, reductions, residual setup   */
-    CURFP_KERNEL_TRSV       = 8,   /* spftrs diagonal-block solves        */
-    CURFP_KERNEL_CLASSES    = 9
+    CURFP_KERNEL_TRSV       = 8,   /* spftrs substitutions                */
+    CURFP_KERNEL_TRSM       = 9,   /* spftrf panel triangular solves      */
+    CURFP_KERNEL_CLASSES    = 10
 } curfpKernelClass_t;
 
 curfpStatus_t curfpSetProfiling(curfpHandle_t handle, int enable);
