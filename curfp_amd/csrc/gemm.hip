@@ -287,14 +287,16 @@ __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
 }
 
 /* C(i, j) = alpha * sum_z P_z(i, j) + beta * C(i, j) over the split-K partials
- * P_z (column-major M x N, leading dimension M, stride M*N). */
+ * P_z (column-major M x N, leading dimension M, stride M*N); a lower output
+ * touches (and its partials hold) only i >= j. */
 template <typename T>
 __global__ void splitk_reduce_kernel(const T *P, int64_t M, int64_t N, int64_t splits,
-                                     T *C, RfpView vc, double alpha, double beta)
+                                     T *C, RfpView vc, double alpha, double beta, int lower)
 {
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= M * N) return;
     const int64_t i = idx % M, j = idx / M;
+    if (lower && i < j) return;
     T s = 0;
     for (int64_t z = 0; z < splits; ++z) s += P[z * M * N + idx];
     T *cp = C + vc.at(i, j);
@@ -339,7 +341,7 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
     if (g.K > 0 && ((!akc && g.va.rs != 1) || (!bkc && g.vb.rs != 1))) return hipErrorInvalidValue;
 
     int64_t splits = 1;
-    if (g.splits > 1 && g.partial && !g.lower && !g.inplace) {
+    if (g.splits > 1 && g.partial && !g.inplace) {
         /* split-K: each slice writes a column-major M x N partial; a second
          * kernel sums them into C with alpha / beta */
         splits = g.splits;
@@ -361,7 +363,7 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
     if (e != hipSuccess || splits == 1 || p.c != g.partial) return e;
     const int64_t total = g.M * g.N;
     hipLaunchKernelGGL((splitk_reduce_kernel<T>), dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s,
-                       (const T *)g.partial, g.M, g.N, splits, (T *)g.c, g.vc, g.alpha, g.beta);
+                       (const T *)g.partial, g.M, g.N, splits, (T *)g.c, g.vc, g.alpha, g.beta, p.lower);
     return hipGetLastError();
 }
 
@@ -405,7 +407,12 @@ hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s)
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     if (g.a_f32) {
         /* FP32-stored operands, FP64 arithmetic (generator residual / norms) */
-        if (g.lower || g.inplace) return hipErrorInvalidValue;
+        if (g.inplace) return hipErrorInvalidValue;
+        if (g.lower) {
+            /* the Gram matrix op(A)^T op(A): square tiles on or below the diagonal */
+            if (!g.b_f32) return hipErrorInvalidValue;
+            return launch_cfg<double, float, float, 128, 128, 2>(g, s);
+        }
         if (g.b_f32) return launch_cfg<double, float, float, 128, 64, 4>(g, s);
         return launch_cfg<double, float, double, 128, 64, 4>(g, s);
     }

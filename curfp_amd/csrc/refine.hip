@@ -52,15 +52,19 @@ __global__ void sumsq_kernel(int64_t n, int64_t nrhs, const double *X, int64_t l
 }
 
 /* out[0] += sum G(i,j)^2, out[1] += trace G, for a k x k column-major G */
+/* ||G||_F^2 and trace(G) of the symmetric k x k G from its lower triangle
+ * (column-major; the strict upper part is not read) */
 __global__ void gram_kernel(int64_t k, const double *G, double *out)
 {
     __shared__ double r0[256], r1[256];
     double s = 0.0, t = 0.0;
     for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < k * k;
          idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % k, j = idx / k;
+        if (i < j) continue;
         const double v = G[idx];
-        s += v * v;
-        if (idx % k == idx / k) t += v;
+        s += (i == j ? 1.0 : 2.0) * v * v;
+        if (i == j) t += v;
     }
     r0[threadIdx.x] = s;
     r1[threadIdx.x] = t;

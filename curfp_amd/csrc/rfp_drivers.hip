@@ -859,7 +859,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
 
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
     const int64_t N = n, NR = nrhs, K = k;
-    const int64_t gsplit = N >= 8192 ? 8 : 1;
+    const int64_t gsplit = N >= 8192 ? 16 : 1;
     /* scratch shared by the solves / residual / Gram product, then the
      * refinement's own buffers */
     size_t shared = pftrs_scratch_bytes(g, false);
@@ -890,6 +890,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
         gg.c = G; gg.vc = RfpView{0, 1, K};
         gg.M = K; gg.N = K; gg.K = N;
         gg.alpha = 1.0; gg.beta = 0.0;
+        gg.lower = true;      /* symmetric: the lower triangle carries both norms */
         gg.a_f32 = true; gg.b_f32 = true;
         gg.splits = (int)gsplit;
         gg.partial = G + K * K;

@@ -291,5 +291,7 @@ hipError_t launch_cast_f64_f32(int64_t n, int64_t nrhs, const double *src, int64
 hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t ldd, double *X, int64_t ldx,
                               bool accumulate, hipStream_t s);
 hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s);
+/* out[0] += ||G||_F^2, out[1] += trace(G) for the symmetric k x k G given by
+ * its lower triangle (column-major) */
 hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_t s);
 hipError_t launch_zero_f64(double *p, int n, hipStream_t s);
