@@ -313,6 +313,32 @@ def test_spftrs_sweep_widths(curfp, n, t, u):
             assert np.all(Bt.cpu().numpy()[:, n:] == 0)
 
 
+@pytest.mark.parametrize("n", [1, 2, 5, 127, 128, 129, 255, 256, 257, 383, 513, 640])
+def test_chain_small_and_boundary_sizes(curfp, n):
+    """Sizes around the 128-row leaves and the 512-column panels, where the
+    panel TRSM, the sweep's block geometry and the look-back products take
+    their ragged paths: the factor against float64 reconstruction and the
+    solve's residual, all 8 variants, row-major and (C ABI) column-major B."""
+    k, nrhs = 24, 3
+    A, B, S = _spd_problem(n, k, nrhs, seed=100 + n)
+    u32 = np.finfo(np.float32).eps
+    for t in "NT":
+        for u in "LU":
+            C = torch.empty(n * (n + 1) // 2, dtype=torch.float32, device="cuda")
+            curfp.ssfrk(dev(A), C, transr=t, uplo=u)
+            curfp.add_to_diagonal(C, 1.0, transr=t, uplo=u)
+            assert curfp.spftrf(C, transr=t, uplo=u) == 0
+            L = O.rfp_lower_factor(C.cpu().numpy().astype(np.float64), n, t, u)
+            assert np.linalg.norm(L @ L.T - S) <= 8 * max(n, 16) * u32 * np.linalg.norm(S), (t, u)
+            X = dev(B)
+            curfp.spftrs(C, X, transr=t, uplo=u)
+            assert O.dense_relative_residual(S, X.cpu().numpy(), B) <= 1e-6, (t, u)
+            Bt = dev(np.ascontiguousarray(B.T))
+            curfp.spftrs_raw(curfp._get_handle(), curfp.OP_T if t == "T" else curfp.OP_N,
+                             curfp.FILL_UPPER if u == "U" else curfp.FILL_LOWER, n, nrhs, C, Bt, n)
+            assert O.dense_relative_residual(S, Bt.cpu().numpy().T, B) <= 1e-6, (t, u)
+
+
 def test_stream_binding(curfp):
     """Operations follow the stream bound with set_stream (test_spftrs.py:169-188)."""
     n, k = 513, 64
