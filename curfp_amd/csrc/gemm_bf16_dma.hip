@@ -32,6 +32,9 @@ namespace {
 
 typedef float  f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+/* 16-byte vector that only assumes element alignment (RFP leading dimensions
+ * leave columns of a view unaligned) */
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 #define LDS_PTR(p) ((__attribute__((address_space(3))) void *)(p))
 
 constexpr int BK = 64;               /* k per slice */
@@ -235,20 +238,40 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
     /* drain the zero-fill loads of the slice beyond K */
     __builtin_amdgcn_s_waitcnt(0);
 
+    /* Epilogue.  A lane holds, for each 32 x 32 block, one column and rows
+     * 4 g + 0..3 (g = 0..3, offset 4 for lanes 32..63): with C a column view
+     * (rows contiguous) each such run of 4 rows is one 16-byte store (and
+     * load when beta != 0) instead of four scattered ones. */
     const bool use_beta = p.beta != 0.f;
+    const bool rows_contig = p.c_rs == 1;
 #pragma unroll
     for (int i = 0; i < F; ++i)
 #pragma unroll
         for (int j = 0; j < F; ++j) {
             const int64_t gj = n0 + wn * (BT / 2) + j * 32 + (lane & 31);
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * (BT / 2) + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
-                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
-                    float val = p.alpha * acc[i][j][r];
-                    if (use_beta) val += p.beta * (*cp);
-                    *cp = val;
+            for (int g = 0; g < 4; ++g) {
+                const int64_t gi0 = m0 + wm * (BT / 2) + i * 32 + 8 * g + 4 * (lane >> 5);
+                if (rows_contig && gi0 + 3 < p.M && gj < p.N && (!p.lower || gi0 >= gj)) {
+                    float *cp = p.c + p.c_off + gi0 + gj * p.c_cs;
+                    f32x4u v = {p.alpha * acc[i][j][4 * g], p.alpha * acc[i][j][4 * g + 1],
+                                p.alpha * acc[i][j][4 * g + 2], p.alpha * acc[i][j][4 * g + 3]};
+                    if (use_beta) {
+                        const f32x4u o = *(const f32x4u *)cp;
+                        v = v + p.beta * o;
+                    }
+                    *(f32x4u *)cp = v;
+                    continue;
+                }
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int64_t gi = gi0 + e;
+                    if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
+                        float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
+                        float val = p.alpha * acc[i][j][4 * g + e];
+                        if (use_beta) val += p.beta * (*cp);
+                        *cp = val;
+                    }
                 }
             }
         }
