@@ -21,6 +21,7 @@
  * loads in flight during the MFMAs.
  */
 #include "rfp_internal.h"
+#include "mfma_epilogue.h"
 
 typedef float  f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -232,23 +233,12 @@ __global__ __launch_bounds__(256) void view_gemm_bf16_kernel(BKParams p)
         __syncthreads();
     }
 
-    const bool use_beta = p.beta != 0.f;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t gj = n0 + wn * 64 + j * 32 + (lane & 31);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
-                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
-                    float val = p.alpha * acc[i][j][r];
-                    if (use_beta) val += p.beta * (*cp);
-                    *cp = val;
-                }
-            }
-        }
+        for (int j = 0; j < 2; ++j)
+            store_block32(acc[i][j], p.c, p.c_off, p.c_rs, p.c_cs, m0 + wm * 64 + i * 32, n0 + wn * 64 + j * 32,
+                          p.M, p.N, p.lower != 0, p.alpha, p.beta, lane);
 }
 
 template <typename S, bool X3>
@@ -421,23 +411,12 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
         __syncthreads();
     }
 
-    const bool use_beta = p.beta != 0.f;
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FM; ++j) {
-            const int64_t gj = n0 + wn * WT + j * 32 + (lane & 31);
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t gi = m0 + wm * WT + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
-                    float *cp = p.c + p.c_off + gi * p.c_rs + gj * p.c_cs;
-                    float val = p.alpha * acc[i][j][r];
-                    if (use_beta) val += p.beta * (*cp);
-                    *cp = val;
-                }
-            }
-        }
+        for (int j = 0; j < FM; ++j)
+            store_block32(acc[i][j], p.c, p.c_off, p.c_rs, p.c_cs, m0 + wm * WT + i * 32, n0 + wn * WT + j * 32,
+                          p.M, p.N, p.lower != 0, p.alpha, p.beta, lane);
 }
 
 template <int BT, bool X3, int BK>
