@@ -82,6 +82,35 @@ __global__ void gram_kernel(int64_t k, const double *G, double *out)
     }
 }
 
+/* the same from an FP32 G (norms accumulated in FP64) */
+__global__ void gram_f32_kernel(int64_t k, const float *G, double *out)
+{
+    __shared__ double r0[256], r1[256];
+    double s = 0.0, t = 0.0;
+    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < k * k;
+         idx += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = idx % k, j = idx / k;
+        if (i < j) continue;
+        const double v = (double)G[idx];
+        s += (i == j ? 1.0 : 2.0) * v * v;
+        if (i == j) t += v;
+    }
+    r0[threadIdx.x] = s;
+    r1[threadIdx.x] = t;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        atomicAdd(out, r0[0]);
+        atomicAdd(out + 1, r1[0]);
+    }
+}
+
 __global__ void zero_f64_kernel(double *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
@@ -116,6 +145,12 @@ hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t l
 hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(sumsq_kernel, dim3(blocks_for(n * nrhs, 2048)), dim3(256), 0, s, n, nrhs, X, ldx, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gram_norms_f32(int64_t k, const float *G, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(gram_f32_kernel, dim3(blocks_for(k * k, 1024)), dim3(256), 0, s, k, G, out);
     return hipGetLastError();
 }
 

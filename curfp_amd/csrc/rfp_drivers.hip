@@ -879,10 +879,13 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     const RfpView dv{0, 1, N};
     hipStream_t s = h->stream;
 
-    /* ||C||_F^2 = a^2 ||A^T A||_F^2 + 2 a lam ||A||_F^2 + n lam^2, via G = op(A)^T op(A) */
+    /* ||C||_F^2 = a^2 ||A^T A||_F^2 + 2 a lam ||A||_F^2 + n lam^2, via the lower
+     * triangle of G = op(A)^T op(A).  ||C|| only normalises the stopping test,
+     * so G is formed in FP32 (relative error of order sqrt(n) u) on the FP32
+     * MFMA path, split-K over n; the norms are summed in FP64. */
     double normC2 = (double)N * lam * lam;
     if (K > 0 && a != 0.0) {
-        double *G = (double *)base;
+        float *G = (float *)base;
         const RfpView av = trans == CURFP_OP_N ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
         GemmArgs gg = gemm_args();
         gg.a = A; gg.va = av.t();
@@ -890,13 +893,12 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
         gg.c = G; gg.vc = RfpView{0, 1, K};
         gg.M = K; gg.N = K; gg.K = N;
         gg.alpha = 1.0; gg.beta = 0.0;
-        gg.lower = true;      /* symmetric: the lower triangle carries both norms */
-        gg.a_f32 = true; gg.b_f32 = true;
+        gg.lower = true;
         gg.splits = (int)gsplit;
         gg.partial = G + K * K;
         if ((e = launch_zero_f64(sums, 4, s)) != hipSuccess) return st_of(e);
-        if ((e = run_gemm<double>(h, gg, RFP_MATH_F64)) != hipSuccess) return st_of(e);
-        if ((e = launch_gram_norms(K, G, sums + 2, s)) != hipSuccess) return st_of(e);
+        if ((e = run_gemm<float>(h, gg, RFP_MATH_F32)) != hipSuccess) return st_of(e);
+        if ((e = launch_gram_norms_f32(K, G, sums + 2, s)) != hipSuccess) return st_of(e);
         double hs[2];
         if ((e = hipMemcpyAsync(hs, sums + 2, sizeof(hs), hipMemcpyDeviceToHost, s)) != hipSuccess) return st_of(e);
         if ((e = hipStreamSynchronize(s)) != hipSuccess) return st_of(e);

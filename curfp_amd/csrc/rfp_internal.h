@@ -294,4 +294,5 @@ hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, d
 /* out[0] += ||G||_F^2, out[1] += trace(G) for the symmetric k x k G given by
  * its lower triangle (column-major) */
 hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_t s);
+hipError_t launch_gram_norms_f32(int64_t k, const float *G, double *out, hipStream_t s);
 hipError_t launch_zero_f64(double *p, int n, hipStream_t s);
