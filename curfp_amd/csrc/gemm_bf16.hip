@@ -453,13 +453,14 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
         if (g.lower && g.M != g.N) return hipErrorInvalidValue;
         const bool x3 = math == RFP_MATH_BF16X3;
         if (x3 && (!g.a_lo || !g.b_lo)) return hipErrorInvalidValue;
-        /* 256 x 256 tiles for the large updates; 64 x 64 tiles (BK = 64) for
-         * problems of fewer than 128 tiles of 128 x 128, where a workgroup's
-         * own latency sets the time (four times the workgroups, a quarter of
-         * the work and half the k-slices each) */
+        /* 256 x 256 tiles for the large updates (M, N >= 4096); 64 x 64
+         * tiles (BK = 64) for problems of fewer than 256 tiles of 128 x 128,
+         * where a workgroup's own latency sets the time (four times the
+         * workgroups, a quarter of the work and half the k-slices each).
+         * Both thresholds measured on the spftrf chain at n = 65536. */
         const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
-        const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) < 128;
-        const bool big = g.M >= 2048 && g.N >= 2048;
+        const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) < 256;
+        const bool big = g.M >= 4096 && g.N >= 4096;
         if (x3) return small ? launch_rows<64, true, 64>(g, s) : launch_rows<128, true, 32>(g, s);
         if (small) return launch_rows<64, false, 64>(g, s);
         const hipError_t e = launch_gemm_bf16_dma(g, big ? 256 : 128, s);

@@ -50,12 +50,13 @@ def test_ssfrk_bf16_input_is_exact_products(curfp, t, u, n, k):
     assert np.linalg.norm(Cd - S) <= 2 * math.sqrt(k) * u32 * np.linalg.norm(S) + 1e-30
 
 
-@pytest.mark.parametrize("n,k,ld,t,u,beta", [(4500, 1000, 1000, "T", "U", 0.0), (4097, 520, 520, "N", "L", 0.5),
+@pytest.mark.parametrize("n,k,ld,t,u,beta", [(8200, 136, 136, "T", "U", 0.0), (8193, 520, 520, "N", "L", 0.5),
+                                              (4500, 1000, 1000, "T", "U", 0.0), (4097, 520, 520, "N", "L", 0.5),
                                               (4300, 1004, 1008, "T", "L", 0.0), (4100, 64, 64, "N", "U", -1.0),
                                               (3001, 1000, 1000, "N", "U", 0.25), (3333, 136, 136, "T", "L", 0.0)])
 def test_ssfrk_bf16_input_large_tiles(curfp, n, k, ld, t, u, beta):
     """Sizes whose RFP blocks reach the BF16 tile kernels for larger problems:
-    256 x 256 (blocks >= 2048 rows) and 128 x 128 (n ~ 3000) with direct-to-LDS
+    256 x 256 (blocks >= 4096 rows, n >= 8192) and 128 x 128 with direct-to-LDS
     loads when K % 8 == 0, the register-staged kernel otherwise (k = 1004
     inside rows of 1008); partial tiles at every edge, beta != 0.
     Products of BF16 values are exact in FP32, so C matches the float64
