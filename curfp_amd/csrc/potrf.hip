@@ -325,12 +325,15 @@ __device__ __forceinline__ void mma(f32x16_t &acc, const float *A, int ai, int a
                                     int lane)
 {
     const int r = lane & 31, h = lane >> 5;
+    /* all operands first: one LDS round trip for the 16 dependent MFMAs */
+    float a[BS / 2], b[BS / 2];
 #pragma unroll
-    for (int k0 = 0; k0 < BS; k0 += 2) {
-        const float a = A[r * ai + (k0 + h) * ak];
-        const float b = B[(k0 + h) * bk + r * bj];
-        acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc, 0, 0, 0);
+    for (int s = 0; s < BS / 2; ++s) {
+        a[s] = A[r * ai + (2 * s + h) * ak];
+        b[s] = B[(2 * s + h) * bk + r * bj];
     }
+#pragma unroll
+    for (int s = 0; s < BS / 2; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
 }
 
 __device__ __forceinline__ void zero(f32x16_t &acc)
@@ -366,41 +369,40 @@ __device__ __forceinline__ void wave_sync()
 }
 
 /*
- * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, one row
- * per lane (lanes 32..63 duplicate lanes 0..31 and store nothing), write L
- * into D's lower triangle and inv(L) into Xd's lower triangle.  W is a
- * 16-byte aligned LDS scratch of BS floats.  Returns the failing local
- * column or -1 (wave-uniform).
+ * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, write L
+ * into D's lower triangle and inv(L) into Xd's lower triangle.  Lane i < 32
+ * holds row i of the block; lane 32 + c holds column c of X = inv(L),
+ * starting from e_c.  W is a 16-byte aligned LDS scratch of BS floats.
+ * Returns the failing local column or -1 (wave-uniform).
  *
- * Right-looking: at column j every lane publishes its entry of the column
- * in LDS and reads the whole column back as 16-byte broadcasts
- * c[k] = A(k, j), so lane i applies  r[k] -= (L(i,j) / L(j,j)) * c[k]  to its
- * row with no cross-lane register traffic.  The update is applied to all
- * entries of a row without a per-entry predicate: entries right of the
- * diagonal (k > i) absorb updates but are never read (step j reads c[k]
- * only for k >= j, i.e. rows at or below the diagonal), so the block's upper
- * triangle in LDS holds garbage afterwards, which no later step reads.
- * 1/sqrt(pivot) is v_rsq_f32 (1 ulp); the pivot's own lane computes
- * L(j,j) = pivot * rsq(pivot).
- * The inverse rides along: lane c also holds column c of X = inv(L),
- * starting from e_c, and applies the same column-j step to it
- * (x[j] /= L(j,j), then x[k] -= L(k,j) x[j] for k > j), so its entries above
- * the diagonal come out as exact zeros.
+ * Right-looking: at column j the row lanes publish their entry of the column
+ * in LDS and every lane reads the whole column back as 16-byte broadcasts
+ * c[k] = A(k, j).  Both kinds of lane then apply the same step to their
+ * vector v:
+ *     v[j] *= 1 / sqrt(A(j, j));   v[k] -= (v[j] / sqrt(A(j, j))) c[k],  k > j
+ * which for a row lane is  L(i, j) = A(i, j) / L(j, j),
+ * A(i, k) -= L(i, j) L(k, j), and for an inverse lane the column step of
+ * forward substitution (x[j] /= L(j, j), x[k] -= L(k, j) x[j]), so one
+ * instruction stream updates the factor rows and the inverse columns
+ * together.  The update is applied to all entries without a per-entry
+ * predicate: row entries right of the diagonal (k > i) absorb updates but
+ * are never read (step j reads c[k] only for k >= j, i.e. rows at or below
+ * the diagonal), so the block's upper triangle in LDS holds garbage
+ * afterwards, which no later step reads; the inverse's entries above the
+ * diagonal come out as exact zeros.  1/sqrt(pivot) is v_rsq_f32 (1 ulp).
  */
 __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
 {
     const int i = lane & 31;
-    const bool own = lane < 32;
-    float r[BS], x[BS];
+    const bool row_lane = lane < 32;
+    float v[BS];
     const float fi = (float)i;
 #pragma unroll
-    for (int k = 0; k < BS; ++k) {
-        r[k] = D[i * LP + k];
-        x[k] = fmaxf(0.f, 1.f - fabsf(fi - (float)k));   /* delta(k, i) */
-    }
+    for (int k = 0; k < BS; ++k)
+        v[k] = row_lane ? D[i * LP + k] : fmaxf(0.f, 1.f - fabsf(fi - (float)k));   /* row i, or e_i */
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
-        if (own) W[i] = r[j];
+        if (row_lane) W[i] = v[j];
         wave_sync();
         float c[BS];
 #pragma unroll
@@ -412,23 +414,15 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
         const float piv = c[j];
         if (!(piv > 0.f)) return j;
         const float rinv = __builtin_amdgcn_rsqf(piv);
-        const float lij = r[j] * rinv;
-        const float t = lij * rinv;
-        r[j] = lij;
-        x[j] *= rinv;
-        const float u = x[j] * rinv;
+        v[j] *= rinv;
+        const float t = v[j] * rinv;
 #pragma unroll
-        for (int k = j + 1; k < BS; ++k) {
-            r[k] = fmaf(-t, c[k], r[k]);
-            x[k] = fmaf(-u, c[k], x[k]);
-        }
+        for (int k = j + 1; k < BS; ++k) v[k] = fmaf(-t, c[k], v[k]);
     }
-    if (own) {
 #pragma unroll
-        for (int k = 0; k < BS; ++k) {
-            D[i * LP + k] = r[k];
-            Xd[k * LP + i] = x[k];
-        }
+    for (int k = 0; k < BS; ++k) {
+        if (row_lane) D[i * LP + k] = v[k];
+        else          Xd[k * LP + i] = v[k];
     }
     return -1;
 }
