@@ -101,9 +101,10 @@ def run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, tol, max_iter, ev=None):
         if ev:
             ev[1].record()
         info = curfp.spftrf(C, check=False)
-    if ev:
-        ev[2].record()
-    X, hist = refine.refine_solve(C, A, lam, B0, tol=tol, max_iter=max_iter)
+        if ev:
+            ev[2].record()
+        # BF16 mode also makes the refinement's sweeps use BF16 products
+        X, hist = refine.refine_solve(C, A, lam, B0, tol=tol, max_iter=max_iter)
     if ev:
         ev[3].record()
     return info, X, hist

@@ -648,29 +648,38 @@ static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
 }
 
 /* Forward + backward substitution on B (view bv, n x nrhs) with the RFP
- * factor A; `inv` holds pftrs_scratch_bytes of scratch for the leaf inverses. */
+ * factor A; `inv` holds pftrs_scratch_bytes of scratch for the leaf inverses.
+ * `prepared`: the inverses (and the sweep's products) in `inv` are already
+ * those of A from an earlier call with the same nrhs path.  `bf16`: BF16
+ * off-diagonal products in the FP32 sweeps (column-major B only). */
 template <typename T>
-static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B, RfpView bv, int64_t nrhs, T *inv)
+static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B, RfpView bv, int64_t nrhs, T *inv,
+                             bool prepared = false, bool bf16 = false)
 {
     const int64_t leaf = rfp_leaf(Prec<T>::f64);
     const int64_t s1 = (g.n1 + leaf - 1) / leaf;
-    RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
-                  [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
-    RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
-                  [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
+    if (!prepared) {
+        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
+                      [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
+        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
+                      [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
+    }
     if (!Prec<T>::f64 && nrhs <= 64 * SWEEP_MAX_GROUPS) {
         /* FP32: one sweep launch per direction over all 128-row blocks */
         float *pm = (float *)(inv + pftrs_inv_elems(g, false)) + ((size_t)8 << 20);
         const double nb = (double)((g.n1 + leaf - 1) / leaf + (g.n2 + leaf - 1) / leaf);
-        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, 4.0 * nb * 2.0 * leaf * leaf * leaf, 0.0, [&] {
-            return launch_sweep_pmat_f32((const float *)A, g, (const float *)inv, pm, h->stream);
-        }));
+        if (!prepared) {
+            RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, 4.0 * nb * 2.0 * leaf * leaf * leaf, 0.0, [&] {
+                return launch_sweep_pmat_f32((const float *)A, g, (const float *)inv, pm, h->stream);
+            }));
+        }
         const double fl = (double)g.n * (double)g.n * (double)nrhs;
+        const bool bf = bf16 && bv.rs == 1;
         for (int dir = 0; dir < 2; ++dir) {
             const unsigned long long ep = ++h->sweep_epoch;
             RFP_TRY(timed(h, CURFP_KERNEL_TRSV, fl, 0.0, [&] {
                 return launch_sweep_solve_f32((const float *)A, g, (const float *)inv, pm, (float *)B, bv,
-                                              (int)nrhs, dir == 1, h->sweep_sync, ep, h->stream);
+                                              (int)nrhs, dir == 1, bf, h->sweep_sync, ep, h->stream);
             }));
         }
         return hipSuccess;
@@ -863,21 +872,25 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
     const int64_t N = n, NR = nrhs, K = k;
     const int64_t gsplit = N >= 8192 ? 16 : 1;
-    /* scratch shared by the solves / residual / Gram product, then the
-     * refinement's own buffers */
-    size_t shared = pftrs_scratch_bytes(g, false);
-    shared = std::max(shared, residual_scratch_bytes(N, K, NR));
-    shared = std::max(shared, (size_t)K * K * (1 + gsplit) * sizeof(double));
+    /* scratch shared by the residual and the Gram product, the solves'
+     * (leaf inverses and sweep products, computed once and kept across the
+     * solves), then the refinement's own buffers */
+    size_t shared = std::max(residual_scratch_bytes(N, K, NR), (size_t)K * K * (1 + gsplit) * sizeof(double));
     shared = align256(shared);
+    const size_t solve_bytes = align256(pftrs_scratch_bytes(g, false));
     const size_t r_bytes = align256((size_t)N * NR * sizeof(double));
     const size_t d_bytes = align256((size_t)N * NR * sizeof(float));
     void *ws = nullptr;
-    hipError_t e = rfp_workspace(h, shared + r_bytes + d_bytes + 256, &ws);
+    hipError_t e = rfp_workspace(h, shared + solve_bytes + r_bytes + d_bytes + 256, &ws);
     if (e != hipSuccess) return st_of(e);
     char *base = (char *)ws;
-    double *R = (double *)(base + shared);
-    float *D = (float *)(base + shared + r_bytes);
-    double *sums = (double *)(base + shared + r_bytes + d_bytes);  /* [0]=||R||^2 [1]=||X||^2 [2..3]=Gram */
+    float *sv = (float *)(base + shared);
+    double *R = (double *)(base + shared + solve_bytes);
+    float *D = (float *)(base + shared + solve_bytes + r_bytes);
+    double *sums = (double *)(base + shared + solve_bytes + r_bytes + d_bytes);  /* [0]=||R||^2 [1]=||X||^2 [2..3]=Gram */
+    /* BF16 mode: the factor is BF16-accurate, so the sweeps' off-diagonal
+     * products run in BF16 too; the refinement absorbs their rounding */
+    const bool bf16_solve = math_of(h) == RFP_MATH_BF16;
     const double a = *alpha, lam = *lambda;
     const RfpView dv{0, 1, N};
     hipStream_t s = h->stream;
@@ -913,7 +926,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     int it = 0;
     double rel = INFINITY;
     if ((e = launch_cast_f64_f32(N, NR, B, ldb, D, N, s)) != hipSuccess) return st_of(e);
-    if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
+    if ((e = pftrs_core<float>(h, g, F, D, dv, NR, sv, false, bf16_solve)) != hipSuccess) return st_of(e);
     if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, false, s)) != hipSuccess) return st_of(e);
     it = 1;
     double best_rn = INFINITY;
@@ -939,7 +952,7 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
         best_rn = std::min(best_rn, rn);
         /* D = F^{-1} fp32(R); X += D */
         if ((e = launch_cast_f64_f32(N, NR, R, N, D, N, s)) != hipSuccess) return st_of(e);
-        if ((e = pftrs_core<float>(h, g, F, D, dv, NR, (float *)base)) != hipSuccess) return st_of(e);
+        if ((e = pftrs_core<float>(h, g, F, D, dv, NR, sv, true, bf16_solve)) != hipSuccess) return st_of(e);
         if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, true, s)) != hipSuccess) return st_of(e);
         ++it;
     }

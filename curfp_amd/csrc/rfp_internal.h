@@ -262,13 +262,16 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
  * the products written by launch_sweep_pmat_f32 at pm.
  * `sync` holds SWEEP_SYNC_BYTES of device words owned by the handle (zeroed
  * once at creation); `epoch` must grow by one per launch on that handle.
- * Supports nrhs <= 64 * SWEEP_MAX_GROUPS.
+ * Supports nrhs <= 64 * SWEEP_MAX_GROUPS.  bf16: the off-diagonal
+ * contributions (all but the last two blocks' of each row) are BF16 products
+ * with FP32 accumulation — for solves inside iterative refinement of a
+ * BF16-accurate factor; needs column-major B (bv.rs == 1).
  */
 #define SWEEP_MAX_GROUPS 64
 #define SWEEP_SYNC_BYTES 1024
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
-                                  RfpView bv, int nrhs, bool trans, void *sync, unsigned long long epoch,
-                                  hipStream_t s);
+                                  RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
+                                  unsigned long long epoch, hipStream_t s);
 /* The sweep's products with the diagonal inverses, 4 * (blocks) 128 x 128
  * matrices at pm (written by this launch, read by both sweeps). */
 hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *inv, float *pm, hipStream_t s);

@@ -399,6 +399,50 @@ __device__ __forceinline__ void mma_sweep(f32x4v (&acc)[2][CT], const float *As,
     }
 }
 
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8v round_bf16(const f32x4v &lo, const f32x4v &hi)
+{
+    return bf16x8v{(__bf16)lo[0], (__bf16)lo[1], (__bf16)lo[2], (__bf16)lo[3],
+                   (__bf16)hi[0], (__bf16)hi[1], (__bf16)hi[2], (__bf16)hi[3]};   /* nearest even */
+}
+
+/*
+ * acc += A (128 x 128) * Y with both operands rounded to BF16 and
+ * v_mfma_f32_16x16x32_bf16 (FP32 accumulation), reading the same FP32 LDS
+ * layouts as mma_sweep.  A 32-wide k chunk p is the FP32 layout's 16-wide
+ * chunks 2p and 2p + 1: lane group lk supplies k = 32 p + 4 lk + s and
+ * 32 p + 16 + 4 lk + s (s < 4) for both operands, so the pairing of A's and
+ * Y's k is the same and the LDS reads stay those of mma_sweep (conflict
+ * free).  The next chunk's FP32 operands are read before the current
+ * chunk's MFMAs and rounded after them.
+ */
+template <int CT, bool AKC>
+__device__ __forceinline__ void mma_sweep_bf16(f32x4v (&acc)[2][CT], const float *As, const float *Ys, int wave,
+                                               int lane)
+{
+    bf16x8v a[2], b[CT];
+    ChunkOps<CT> lo, hi;
+    chunk_load<CT, AKC>(lo, As, Ys, 0, wave, lane);
+    chunk_load<CT, AKC>(hi, As, Ys, 1, wave, lane);
+#pragma unroll
+    for (int p = 0; p < LF / 32; ++p) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) a[t] = round_bf16(lo.a[t], hi.a[t]);
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct) b[ct] = round_bf16(lo.b[ct], hi.b[ct]);
+        if (p + 1 < LF / 32) {
+            chunk_load<CT, AKC>(lo, As, Ys, 2 * p + 2, wave, lane);
+            chunk_load<CT, AKC>(hi, As, Ys, 2 * p + 3, wave, lane);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+                acc[t][ct] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[ct], acc[t][ct], 0, 0, 0);
+    }
+}
+
 /*
  * The products P_b,l that take the diagonal inverse off the sweep's serial
  * chain (one workgroup per block b, direction and l = 1, 2):
@@ -498,8 +542,11 @@ constexpr int SWEEP_SPIN_LIMIT = 1 << 20;
  * that leaves a whole step of slack for the rest.  The operand of step j + 2 (an L tile, the inverse or P_b) is
  * loaded into registers while step j runs, and the next step's Y block (or
  * B_b) as soon as it is known to be finished.
+ * BF16: the contributions of blocks 0 .. o-3 are BF16 products
+ * (mma_sweep_bf16); the diagonal inverse and the look-back products stay
+ * FP32.
  */
-template <int CW, bool TRANS, bool YCM>
+template <int CW, bool TRANS, bool YCM, bool BF16>
 __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
 {
     constexpr int CT = CW / 16;
@@ -636,8 +683,13 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             else   { kc0 = bk.cs == 1; st[0].load(bk, tid); }
         }
         if (j < zs) {
-            if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
-            else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
+            if (BF16) {
+                if (kc) mma_sweep_bf16<CT, true>(acc, As, Ys, wave, lane);
+                else    mma_sweep_bf16<CT, false>(acc, As, Ys, wave, lane);
+            } else {
+                if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
+                else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
+            }
         } else if (j == zs) {
             /* R = B_b - acc in place in Ys, then acc = inv(L_bb) R (transposed backward) */
 #pragma unroll
@@ -754,8 +806,8 @@ hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *
 }
 
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
-                                  RfpView bv, int nrhs, bool trans, void *sync, unsigned long long epoch,
-                                  hipStream_t s)
+                                  RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
+                                  unsigned long long epoch, hipStream_t s)
 {
     if (g.n <= 0 || nrhs <= 0) return hipSuccess;
     const int cw = nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : 64;
@@ -770,11 +822,15 @@ hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float 
     a.epoch = epoch;
     const unsigned grid = (unsigned)a.nb * (unsigned)groups;
     const bool ycm = bv.rs == 1;
-#define SWEEP_LAUNCH(CW_, T_, C_) hipLaunchKernelGGL((sweep_solve_kernel<CW_, T_, C_>), dim3(grid), dim3(256), 0, s, a)
-#define SWEEP_LAUNCH_CW(CW_)                                                \
-    do {                                                                    \
-        if (trans) { if (ycm) SWEEP_LAUNCH(CW_, true, true);  else SWEEP_LAUNCH(CW_, true, false); }   \
-        else       { if (ycm) SWEEP_LAUNCH(CW_, false, true); else SWEEP_LAUNCH(CW_, false, false); }  \
+    /* the BF16 products are built for column-major right-hand sides only */
+    if (bf16 && !ycm) return hipErrorInvalidValue;
+#define SWEEP_LAUNCH(CW_, T_, C_, B_) \
+    hipLaunchKernelGGL((sweep_solve_kernel<CW_, T_, C_, B_>), dim3(grid), dim3(256), 0, s, a)
+#define SWEEP_LAUNCH_CW(CW_)                                                                         \
+    do {                                                                                             \
+        if (bf16) { if (trans) SWEEP_LAUNCH(CW_, true, true, true); else SWEEP_LAUNCH(CW_, false, true, true); } \
+        else if (trans) { if (ycm) SWEEP_LAUNCH(CW_, true, true, false);  else SWEEP_LAUNCH(CW_, true, false, false); } \
+        else            { if (ycm) SWEEP_LAUNCH(CW_, false, true, false); else SWEEP_LAUNCH(CW_, false, false, false); } \
     } while (0)
     if (cw == 16) SWEEP_LAUNCH_CW(16);
     else if (cw == 32) SWEEP_LAUNCH_CW(32);

@@ -199,6 +199,9 @@ curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelClass_t cls,
  *   X_0 = F^{-1} B;  repeat  R = B - C X (FP64, from the generator),
  *   X += F^{-1} fp32(R)  (FP64 accumulation)
  * until ||R||_F <= tol * ||C||_F * ||X||_F or max_iter solves.
+ * Under CURFP_MATH_BF16 (the handle's mode at this call) the solves' FP32
+ * substitution sweeps take their off-diagonal products in BF16 (FP32
+ * accumulation), matching the factor's BF16 accuracy; other modes solve in FP32.
  * B and X are FP64 column-major n x nrhs (ldb, ldx >= n); X need not be
  * initialised.  *iters = solves done, *resid = final relative residual.
  * Host-synchronous (reads the residual norm once per iteration).

@@ -196,3 +196,39 @@ def test_refinement_stops_when_diverging(curfp):
     assert info["solves"] <= 6, info
     assert info["rel_residual"] > 1e-6
     assert bool(torch.isfinite(X).all())
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("nrhs", [1, 17, 64, 100])
+def test_refinement_bf16_solves(curfp, t, u, nrhs):
+    """Refinement under the BF16 math mode runs the sweeps' off-diagonal
+    products in BF16.  Checked: (1) the first solve is a solve with the
+    computed factor F = L L^T to BF16 backward error — ||F X0 - B|| /
+    (||F|| ||X0||) within a small multiple of 2^-8 — and differs from the FP32
+    sweep's X0 (so the BF16 products ran); (2) the refinement still reaches
+    1e-6, verified in float64 on the host, within 4 solves."""
+    n, k, lam = 1000 if t == "N" else 1153, 200, 1.0
+    A, B = problem(n, k, nrhs, seed=33 + nrhs)
+    Ad = torch.from_numpy(A).cuda()
+    Bd = torch.from_numpy(B).cuda()
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    from curfp_amd import refine
+
+    with curfp.math_mode("bf16"):
+        curfp.ssfrk(Ad, C, transr=t, uplo=u)
+        curfp.add_to_diagonal(C, lam, transr=t, uplo=u)
+        assert curfp.spftrf(C, transr=t, uplo=u) == 0
+        X0b, _ = refine.refine_solve(C, Ad, lam, Bd, tol=0.0, max_iter=1, transr=t, uplo=u)
+        X, info = refine.refine_solve(C, Ad, lam, Bd, tol=1e-6, transr=t, uplo=u)
+    X0f, _ = refine.refine_solve(C, Ad, lam, Bd, tol=0.0, max_iter=1, transr=t, uplo=u)
+    L = O.rfp_lower_factor(C.cpu().numpy().astype(np.float64), n, t, u)
+    F = L @ L.T
+    x0b, x0f = X0b.cpu().numpy(), X0f.cpu().numpy()
+    back = np.linalg.norm(F @ x0b - B) / (np.linalg.norm(F) * np.linalg.norm(x0b))
+    assert back <= 4 * 2.0 ** -8, back
+    d = np.linalg.norm(x0b - x0f) / np.linalg.norm(x0f)
+    assert 1e-7 <= d <= 5e-2, d
+    S = A.astype(np.float64) @ A.astype(np.float64).T + lam * np.eye(n)
+    r = O.dense_relative_residual(S, X.cpu().numpy(), B.astype(np.float64))
+    assert info["rel_residual"] <= 1e-6 and r <= 1.5e-6, (info, r)
+    assert info["solves"] <= 4, info
