@@ -442,7 +442,7 @@ struct Factor {
             const double by = (double)mb * m * 4.0 * 2.0 + (double)m * m * 4.0 + (sh_hi ? (double)mb * m * 2.0 : 0.0);
             return timed(h, CURFP_KERNEL_TRSM, fl, by, [&] {
                 return launch_trsm_panel_f32((float *)a, B, mb, L, m, (const float *)(inv + slot * leaf * leaf),
-                                             sh_hi, sh_lo, Bs, h->stream);
+                                             sh_hi, sh_lo, Bs, math == RFP_MATH_BF16, h->stream);
             });
         }
         if (m <= leaf)

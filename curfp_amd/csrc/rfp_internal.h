@@ -249,11 +249,12 @@ hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const f
  * and L an m x m lower-triangular view with m <= trsm_panel_max() (at most
  * four 128-row leaves, whose inverses are at inv + q*128*128, row-major).
  * When sh_hi is set the result is also rounded into the BF16 shadow view S
- * (and its low part into sh_lo).
+ * (and its low part into sh_lo).  bf16: the products with off-diagonal
+ * leaves of L are BF16 products with FP32 accumulation (BF16 math mode).
  */
 int trsm_panel_max();
 hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int64_t m, const float *inv,
-                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, hipStream_t s);
+                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, bool bf16, hipStream_t s);
 
 /*
  * FP32 substitution over the whole RFP factor in one launch (trsv_block.hip):

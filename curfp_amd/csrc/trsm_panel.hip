@@ -101,6 +101,47 @@ __device__ __forceinline__ void mma_panel(f32x4v (&acc)[2][2], const float *Xs, 
     }
 }
 
+typedef __bf16 bf16x8v __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8v round_bf16(const f32x4v &lo, const f32x4v &hi)
+{
+    return bf16x8v{(__bf16)lo[0], (__bf16)lo[1], (__bf16)lo[2], (__bf16)lo[3],
+                   (__bf16)hi[0], (__bf16)hi[1], (__bf16)hi[2], (__bf16)hi[3]};   /* nearest even */
+}
+
+/* mma_panel (no triangular skip) with both operands rounded to BF16 and
+ * v_mfma_f32_16x16x32_bf16: a 32-wide k chunk p is the FP32 chunks 2p and
+ * 2p + 1, lane group lk supplying k = 32 p + 4 lk + s and 32 p + 16 + 4 lk + s
+ * (s < 4) for both operands, so the LDS reads are those of mma_panel. */
+template <bool BKC>
+__device__ __forceinline__ void mma_panel_bf16(f32x4v (&acc)[2][2], const float *Xs, int x0, const float *Ls,
+                                               int wave, int lane)
+{
+    bf16x8v a[2], b[2];
+    Ops lo, hi;
+    ops_load<BKC>(lo, Xs, x0, Ls, 0, wave, lane);
+    ops_load<BKC>(hi, Xs, x0, Ls, 1, wave, lane);
+#pragma unroll
+    for (int q = 0; q < LF / 32; ++q) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t) a[t] = round_bf16(lo.a[t], hi.a[t]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) b[c] = round_bf16(lo.b[c], hi.b[c]);
+        if (q + 1 < LF / 32) {
+            ops_load<BKC>(lo, Xs, x0, Ls, 2 * q + 2, wave, lane);
+            ops_load<BKC>(hi, Xs, x0, Ls, 2 * q + 3, wave, lane);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c)
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[t], b[c], acc[t][c], 0, 0, 0);
+    }
+}
+
+/* BF16: the off-diagonal products X_p L_qp^T are BF16 products (FP32
+ * accumulation); the inverse products stay FP32. */
+template <bool BF16>
 __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
 {
     __shared__ __attribute__((aligned(16))) float Xs[RM * XP];
@@ -228,8 +269,13 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
             st.load(b, tid);
         }
         if (pp < q) {
-            if (kc) mma_panel<true, false>(acc, Xs, pp * LF, Ls, wave, lane);
-            else    mma_panel<false, false>(acc, Xs, pp * LF, Ls, wave, lane);
+            if (BF16) {
+                if (kc) mma_panel_bf16<true>(acc, Xs, pp * LF, Ls, wave, lane);
+                else    mma_panel_bf16<false>(acc, Xs, pp * LF, Ls, wave, lane);
+            } else {
+                if (kc) mma_panel<true, false>(acc, Xs, pp * LF, Ls, wave, lane);
+                else    mma_panel<false, false>(acc, Xs, pp * LF, Ls, wave, lane);
+            }
         } else {
             /* X_q = R_q inv(L_qq)^T; the slab's leaf q is rewritten once every wave has read it */
             mma_panel<true, true>(acc, Xs, q * LF, Ls, wave, lane);
@@ -310,7 +356,7 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
 int trsm_panel_max() { return MAXQ * LF; }
 
 hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int64_t m, const float *inv,
-                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, hipStream_t s)
+                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, bool bf16, hipStream_t s)
 {
     if (mb <= 0 || m <= 0) return hipSuccess;
     if (m > MAXQ * LF || mb > (int64_t)INT32_MAX) return hipErrorInvalidValue;
@@ -325,6 +371,7 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
     pa.sh_lo = sh_lo;
     pa.S = S;
     const unsigned grid = (unsigned)((mb + RM - 1) / RM);
-    hipLaunchKernelGGL(trsm_panel_kernel, dim3(grid), dim3(256), 0, s, pa);
+    if (bf16) hipLaunchKernelGGL(trsm_panel_kernel<true>, dim3(grid), dim3(256), 0, s, pa);
+    else      hipLaunchKernelGGL(trsm_panel_kernel<false>, dim3(grid), dim3(256), 0, s, pa);
     return hipGetLastError();
 }
