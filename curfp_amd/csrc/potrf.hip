@@ -372,24 +372,27 @@ __device__ __forceinline__ void wave_sync()
  * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, write L
  * into D's lower triangle and inv(L) into Xd's lower triangle.  Lane i < 32
  * holds row i of the block; lane 32 + c holds column c of X = inv(L),
- * starting from e_c.  W is a 16-byte aligned LDS scratch of BS floats.
+ * starting from e_c.  W is a 16-byte aligned LDS scratch of 2 x BS floats.
  * Returns the failing local column or -1 (wave-uniform).
  *
- * Right-looking: at column j the row lanes publish their entry of the column
- * in LDS and every lane reads the whole column back as 16-byte broadcasts
- * c[k] = A(k, j).  Both kinds of lane then apply the same step to their
- * vector v:
+ * Right-looking: column j is published by the row lanes in LDS and read
+ * back by every lane as 16-byte broadcasts c[k] = A(k, j).  Both kinds of
+ * lane then apply the same step to their vector v:
  *     v[j] *= 1 / sqrt(A(j, j));   v[k] -= (v[j] / sqrt(A(j, j))) c[k],  k > j
  * which for a row lane is  L(i, j) = A(i, j) / L(j, j),
  * A(i, k) -= L(i, j) L(k, j), and for an inverse lane the column step of
  * forward substitution (x[j] /= L(j, j), x[k] -= L(k, j) x[j]), so one
  * instruction stream updates the factor rows and the inverse columns
- * together.  The update is applied to all entries without a per-entry
- * predicate: row entries right of the diagonal (k > i) absorb updates but
- * are never read (step j reads c[k] only for k >= j, i.e. rows at or below
- * the diagonal), so the block's upper triangle in LDS holds garbage
- * afterwards, which no later step reads; the inverse's entries above the
- * diagonal come out as exact zeros.  1/sqrt(pivot) is v_rsq_f32 (1 ulp).
+ * together.  Software-pipelined: step j updates v[j + 1] first, publishes
+ * it (column j + 1, final at that point) in the other half of W and issues
+ * its reads before the rest of step j's updates, so the LDS round trip of
+ * the next column overlaps the current one's arithmetic.  The update is
+ * applied to all entries without a per-entry predicate: row entries right of
+ * the diagonal (k > i) absorb updates but are never read (step j reads c[k]
+ * only for k >= j, i.e. rows at or below the diagonal), so the block's upper
+ * triangle in LDS holds garbage afterwards, which no later step reads; the
+ * inverse's entries above the diagonal come out as exact zeros.
+ * 1/sqrt(pivot) is v_rsq_f32 (1 ulp).
  */
 __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
 {
@@ -400,24 +403,37 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
 #pragma unroll
     for (int k = 0; k < BS; ++k)
         v[k] = row_lane ? D[i * LP + k] : fmaxf(0.f, 1.f - fabsf(fi - (float)k));   /* row i, or e_i */
+    /* column jn from W half h into c (entries jn .. BS-1, in 16-byte groups) */
+    auto fetch = [&](float (&c)[BS], int jn, int h) {
+        const float4 *w = reinterpret_cast<const float4 *>(W + h * BS);
 #pragma unroll
-    for (int j = 0; j < BS; ++j) {
-        if (row_lane) W[i] = v[j];
-        wave_sync();
-        float c[BS];
-#pragma unroll
-        for (int g = j / 4; g < BS / 4; ++g) {
-            const float4 t = reinterpret_cast<const float4 *>(W)[g];
+        for (int g = jn / 4; g < BS / 4; ++g) {
+            const float4 t = w[g];
             c[4 * g] = t.x; c[4 * g + 1] = t.y; c[4 * g + 2] = t.z; c[4 * g + 3] = t.w;
         }
-        wave_sync();
-        const float piv = c[j];
+    };
+    float c[2][BS];
+    if (row_lane) W[i] = v[0];
+    wave_sync();
+    fetch(c[0], 0, 0);
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float (&cc)[BS] = c[j & 1];
+        const float piv = cc[j];
         if (!(piv > 0.f)) return j;
         const float rinv = __builtin_amdgcn_rsqf(piv);
         v[j] *= rinv;
         const float t = v[j] * rinv;
+        if (j + 1 < BS) {
+            v[j + 1] = fmaf(-t, cc[j + 1], v[j + 1]);
+            /* half (j+1)&1 was last read at step j-1, whose reads this wave
+             * has completed (they fed step j-1's updates) */
+            if (row_lane) W[((j + 1) & 1) * BS + i] = v[j + 1];
+            wave_sync();
+            fetch(c[(j + 1) & 1], j + 1, (j + 1) & 1);
+        }
 #pragma unroll
-        for (int k = j + 1; k < BS; ++k) v[k] = fmaf(-t, c[k], v[k]);
+        for (int k = j + 2; k < BS; ++k) v[k] = fmaf(-t, cc[k], v[k]);
     }
 #pragma unroll
     for (int k = 0; k < BS; ++k) {
