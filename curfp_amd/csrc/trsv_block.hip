@@ -542,9 +542,10 @@ constexpr int SWEEP_SPIN_LIMIT = 1 << 20;
  * that leaves a whole step of slack for the rest.  The operand of step j + 2 (an L tile, the inverse or P_b) is
  * loaded into registers while step j runs, and the next step's Y block (or
  * B_b) as soon as it is known to be finished.
- * BF16: the contributions of blocks 0 .. o-3 are BF16 products
- * (mma_sweep_bf16); the diagonal inverse and the look-back products stay
- * FP32.
+ * BF16: the contributions of blocks 0 .. o-3 and the look-back products
+ * are BF16 products (mma_sweep_bf16); the diagonal inverse stays FP32.
+ * With column-major B the block is stored straight from the accumulators
+ * (each lane's 4 consecutive rows of a column as one 16-byte store).
  */
 template <int CW, bool TRANS, bool YCM, bool BF16>
 __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
@@ -722,7 +723,8 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             }
         } else {
             /* acc += P_b,l Y_c; after the last, the block's result is Z - acc */
-            mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
+            if (BF16) mma_sweep_bf16<CT, true>(acc, As, Ys, wave, lane);
+            else      mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
             if (j == o) {
 #pragma unroll
                 for (int t = 0; t < 2; ++t)
@@ -737,21 +739,41 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             }
         }
     }
-    lds_barrier();
     const float nan = __builtin_nanf("");
     const bool fail = s_fail != 0;
+    if (YCM) {
+        /* rows contiguous in B: a lane's 4 rows of a column in one store */
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+        for (int t = 0; t < 2; ++t)
 #pragma unroll
-        for (int ct = 0; ct < CT; ++ct)
+            for (int ct = 0; ct < CT; ++ct) {
+                const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4), col = ct * 16 + (lane & 15);
+                if (col >= cols) continue;
+                float *p = a.B + a.bv.off + (grow + row) + (int64_t)(col0 + col) * a.bv.cs;
+                const f32x4v v = fail ? f32x4v{nan, nan, nan, nan} : acc[t][ct];
+                if (row + 4 <= rows) {
+                    *(f32x4u *)p = f32x4u{v[0], v[1], v[2], v[3]};
+                } else {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
-                *YStage<CW, YCM>::lds(Ys, row, col) = fail ? nan : acc[t][ct][r];
+                    for (int r = 0; r < 4; ++r)
+                        if (row + r < rows) p[r] = v[r];
+                }
             }
-    lds_barrier();
-    yv.load_lds(Ys, tid);
-    yv.store(a, grow, rows, col0, cols, tid);
+    } else {
+        lds_barrier();
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
+                    *YStage<CW, YCM>::lds(Ys, row, col) = fail ? nan : acc[t][ct][r];
+                }
+        lds_barrier();
+        yv.load_lds(Ys, tid);
+        yv.store(a, grow, rows, col0, cols, tid);
+    }
     /* publish: every wave's stores are written back before the progress word moves */
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
