@@ -59,6 +59,8 @@ try:
         dstrttf as _dstrttf_C,
         dstfttr as _dstfttr_C,
         dadd_to_diagonal as _dadd_diag_C,
+        slansf as _slansf_C,
+        dlansf as _dlansf_C,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(
@@ -66,10 +68,21 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
         "(hipcc --offload-arch=gfx950) first"
     ) from exc
 
-# Enum values of the reference module (curfp/_ext.cpp:869-877) that belong to
-# routines outside the RFP hot path built here.
+# Enum values of the reference module (curfp/_ext.cpp:869-877): the norm types
+# of slansf / dlansf, and the sides of ssfmm (outside the path built here).
 NORM_MAX, NORM_ONE, NORM_FRO = 0, 1, 2
 SIDE_LEFT, SIDE_RIGHT = 0, 1
+
+# norm strings accepted by slansf / dlansf (curfp/__init__.py:114-123)
+_NORM_MAP = {"M": NORM_MAX, "MAX": NORM_MAX, "1": NORM_ONE, "O": NORM_ONE, "I": NORM_ONE,
+             "F": NORM_FRO, "E": NORM_FRO, "FRO": NORM_FRO}
+
+
+def _norm(s: str) -> int:
+    try:
+        return _NORM_MAP[s.upper()]
+    except KeyError:
+        raise ValueError(f"norm must be 'M'/'1'/'F' (or 'O'/'I'/'E'), got {s!r}") from None
 
 # ---------------------------------------------------------------------------
 # Global handle cache (one handle per device) — curfp/__init__.py:88-106
@@ -500,6 +513,44 @@ def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R
 
 
 # ---------------------------------------------------------------------------
+# Norms of a symmetric RFP matrix — curfp/__init__.py:784-841 (slansf),
+# :1700-1720 (dlansf)
+# ---------------------------------------------------------------------------
+def slansf_raw(handle: Handle, norm: int, transr: int, uplo: int, n: int, C: torch.Tensor) -> float:
+    """Low-level norm of the symmetric matrix stored in C (RFP, float32):
+    integer norm / transr / uplo codes, explicit n."""
+    _validate(C, "C")
+    return float(_slansf_C(handle, norm, transr, uplo, n, C.data_ptr()))
+
+
+def slansf(C: torch.Tensor, norm: str = "1", *, transr: str = "T", uplo: str = "U", n: int = None) -> float:
+    """Norm of a symmetric matrix stored in RFP format (float32 tensor of
+    n*(n+1)/2 elements; the matrix itself, not its Cholesky factor):
+    'M'/'max' max |a_ij|; '1'/'O'/'I' largest absolute column sum (the
+    1-norm, equal to the infinity norm); 'F'/'E'/'fro' Frobenius norm.
+    Sizes are 64-bit (n >= 65536 works; the reference's cuBLAS reductions
+    stop at n(n+1)/2 = INT_MAX)."""
+    _validate(C, "C")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    return float(_slansf_C(_get_handle(), _norm(norm), _op(transr), _fill(uplo), n, C.data_ptr()))
+
+
+def dlansf_raw(handle: Handle, norm: int, transr: int, uplo: int, n: int, C: torch.Tensor) -> float:
+    """FP64 twin of slansf_raw."""
+    _validate(C, "C", torch.float64)
+    return float(_dlansf_C(handle, norm, transr, uplo, n, C.data_ptr()))
+
+
+def dlansf(C: torch.Tensor, norm: str = "1", *, transr: str = "T", uplo: str = "U", n: int = None) -> float:
+    """FP64 twin of slansf."""
+    _validate(C, "C", torch.float64)
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    return float(_dlansf_C(_get_handle(), _norm(norm), _op(transr), _fill(uplo), n, C.data_ptr()))
+
+
+# ---------------------------------------------------------------------------
 # Reference names outside the RFP hot path (curfp/__init__.py:42-80)
 # ---------------------------------------------------------------------------
 def _not_built(name):
@@ -510,8 +561,8 @@ def _not_built(name):
     return fn
 
 
-for _name in ("spftri", "ssfmv", "slansf", "spfcon", "ssfr", "ssfr2", "ssfr2k", "ssfmm", "spftrf_rcond",
-              "dpftri", "dsfmv", "dlansf", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dsfmm", "dpftrf_rcond"):
+for _name in ("spftri", "ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "ssfmm", "spftrf_rcond",
+              "dpftri", "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dsfmm", "dpftrf_rcond"):
     globals()[_name] = _not_built(_name)
     globals()[_name + "_raw"] = _not_built(_name + "_raw")
 del _name

@@ -1,0 +1,296 @@
+/*
+ * lansf.hip — norms of a symmetric matrix stored in RFP: max |a_ij|, the
+ * 1-norm (the largest absolute column sum of the full symmetric matrix, which
+ * is also its infinity norm) and the Frobenius norm (reference
+ * src/curfp_slansf.cu:273-435, src/curfp_dlansf.cu:189).  The reference reduces
+ * with cuBLAS calls that take an int element count and returns
+ * INVALID_VALUE once n(n+1)/2 > INT_MAX (n >= 65536); every index here is
+ * 64-bit.
+ *
+ * max / Frobenius: one grid-stride pass over the packed array (16-byte loads,
+ * FP64 sums), then a pass over the n diagonal entries: ||A||_F^2 =
+ * 2 sum(arf^2) - sum(diag^2).
+ *
+ * 1-norm: the stored triangle L (lower, as the three views) gives column
+ * sums of the full matrix as  s_j = sum_{i >= j} |L_ij| + sum_{k < j} |L_jk|.
+ * One workgroup per 128 x 128 tile (bi, bj), bi >= bj, of the 128-row blocks
+ * of v11 then v22 (the block structure of the substitution sweep) reads the
+ * tile once and writes its column sums to S[bi][cols of bj] and its strict
+ * row sums to S[bj][rows of bi] (a diagonal tile writes both into
+ * S[bi][block bi]).  Every (slot, index) pair of S is written by exactly one
+ * tile, so S needs no clearing and no atomics; a second kernel sums the
+ * slots of each index in FP64 and takes the maximum.
+ */
+#include "rfp_internal.h"
+#include <algorithm>
+
+namespace {
+
+constexpr int NT = 128;       /* tile of the 1-norm */
+constexpr int TPP = NT + 1;   /* LDS pitch */
+
+struct LansfGeom {
+    RfpView v11, v21, v22;
+    int64_t n, n1, n2;
+    int s1, nb;
+};
+
+__device__ __forceinline__ void lansf_block(const LansfGeom &g, int b, int64_t &grow, int64_t &lrow, int &rows)
+{
+    if (b < g.s1) {
+        lrow = (int64_t)b * NT;
+        grow = lrow;
+        rows = (int)min<int64_t>(NT, g.n1 - lrow);
+    } else {
+        lrow = (int64_t)(b - g.s1) * NT;
+        grow = g.n1 + lrow;
+        rows = (int)min<int64_t>(NT, g.n2 - lrow);
+    }
+}
+
+template <typename T> struct Vec16;
+template <> struct Vec16<float> { typedef float type __attribute__((ext_vector_type(4))); static constexpr int N = 4; };
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); static constexpr int N = 2; };
+
+/* the order of non-negative IEEE values is the order of their bit patterns;
+ * NaN's pattern is above +inf, so a NaN entry wins the maximum */
+__device__ __forceinline__ unsigned long long key_of(float x) { return __float_as_uint(x); }
+__device__ __forceinline__ unsigned long long key_of(double x) { return (unsigned long long)__double_as_longlong(x); }
+
+template <typename T>
+__device__ __forceinline__ T absmax(T m, T x)
+{
+    return (x > m || x != x) ? x : m;
+}
+
+/* block reduction of (max, sum) over 256 threads; thread 0 gets the result */
+template <typename T>
+__device__ __forceinline__ void block_reduce(T &m, double &s)
+{
+    __shared__ T sm[256];
+    __shared__ double ss[256];
+    const int tid = threadIdx.x;
+    sm[tid] = m;
+    ss[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) {
+            sm[tid] = absmax(sm[tid], sm[tid + w]);
+            ss[tid] += ss[tid + w];
+        }
+        __syncthreads();
+    }
+    m = sm[0];
+    s = ss[0];
+}
+
+/* acc[0] (bits) = max |a|, acc[1] (double) += sum a^2 over the nt entries */
+template <typename T>
+__global__ __launch_bounds__(256) void absmax_sumsq_kernel(const T *a, int64_t nt, unsigned long long *acc)
+{
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    /* head up to a 16-byte boundary, 16-byte body, tail */
+    const int64_t mis = (int64_t)(((uintptr_t)a / sizeof(T)) % VN);
+    const int64_t head = min<int64_t>(nt, mis ? VN - mis : 0);
+    const int64_t nv = (nt - head) / VN;
+    T m = 0;
+    double s = 0.0;
+    if (tid < head) {
+        const T x = fabs(a[tid]);
+        m = absmax(m, x);
+        s += (double)x * x;
+    }
+    const V *av = reinterpret_cast<const V *>(a + head);
+    for (int64_t i = tid; i < nv; i += stride) {
+        const V v = av[i];
+#pragma unroll
+        for (int e = 0; e < VN; ++e) {
+            const T x = fabs(v[e]);
+            m = absmax(m, x);
+            s += (double)x * x;
+        }
+    }
+    for (int64_t i = head + nv * VN + tid; i < nt; i += stride) {
+        const T x = fabs(a[i]);
+        m = absmax(m, x);
+        s += (double)x * x;
+    }
+    block_reduce(m, s);
+    if (threadIdx.x == 0) {
+        atomicMax(acc, key_of(m));
+        atomicAdd(reinterpret_cast<double *>(acc + 1), s);
+    }
+}
+
+/* acc[2] (double) += sum of the squared diagonal */
+template <typename T>
+__global__ __launch_bounds__(256) void diag_sumsq_kernel(const T *a, LansfGeom g, unsigned long long *acc)
+{
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += (int64_t)gridDim.x * blockDim.x) {
+        const T x = a[i < g.n1 ? g.v11.at(i, i) : g.v22.at(i - g.n1, i - g.n1)];
+        s += (double)x * x;
+    }
+    T m = 0;
+    block_reduce(m, s);
+    if (threadIdx.x == 0) atomicAdd(reinterpret_cast<double *>(acc + 2), s);
+}
+
+/* column / strict row sums of one tile of |L| into S (slot * n + index) */
+template <typename T>
+__global__ __launch_bounds__(256) void colsum_tiles_kernel(const T *a, LansfGeom g, T *S)
+{
+    __shared__ T tile[NT * TPP];
+    __shared__ T csum[NT];
+    const int tid = threadIdx.x;
+    int64_t bi, bj;
+    tri_tile_coords((int64_t)blockIdx.x, g.nb, bi, bj);   /* bi >= bj */
+    int64_t gr, lr, gc, lc;
+    int mr, mc;
+    lansf_block(g, (int)bi, gr, lr, mr);
+    lansf_block(g, (int)bj, gc, lc, mc);
+    const RfpView v = bi < g.s1 ? g.v11.sub(lr, lc) : (bj < g.s1 ? g.v21.sub(lr, lc) : g.v22.sub(lr, lc));
+    const bool diag = bi == bj;
+    const bool rows_contig = v.rs == 1;
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    constexpr int PER = NT * NT / VN / 256;    /* 16-byte pieces per thread */
+    constexpr int RUN = NT / VN;               /* pieces along the contiguous direction */
+    if (!diag && mr == NT && mc == NT) {
+        /* interior tile: every load issued before the LDS stores; piece q
+         * covers VN consecutive entries along the contiguous direction */
+        V x[PER];
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int q = tid + 256 * e;
+            const int a0 = (q % RUN) * VN, b0 = q / RUN;
+            const T *p = a + (rows_contig ? v.at(a0, b0) : v.at(b0, a0));
+            typedef T Vu __attribute__((ext_vector_type(VN), aligned(sizeof(T))));
+            const Vu u = *(const Vu *)p;
+#pragma unroll
+            for (int k = 0; k < VN; ++k) x[e][k] = fabs(u[k]);
+        }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int q = tid + 256 * e;
+            const int a0 = (q % RUN) * VN, b0 = q / RUN;
+#pragma unroll
+            for (int k = 0; k < VN; ++k) {
+                if (rows_contig) tile[(a0 + k) * TPP + b0] = x[e][k];
+                else             tile[b0 * TPP + a0 + k] = x[e][k];
+            }
+        }
+    } else {
+        /* edge or diagonal tile; entries above a diagonal tile's diagonal
+         * belong to other blocks of the packed array and are not read */
+#pragma unroll 8
+        for (int e = 0; e < NT * NT / 256; ++e) {
+            const int idx = tid + 256 * e;
+            const int r = rows_contig ? idx % NT : idx / NT;
+            const int c = rows_contig ? idx / NT : idx % NT;
+            T x = 0;
+            if (r < mr && c < mc && (!diag || r >= c)) x = fabs(a[v.at(r, c)]);
+            tile[r * TPP + c] = x;
+        }
+    }
+    __syncthreads();
+    if (tid < NT) {
+        const int c = tid;
+        T s = 0;
+        for (int r = 0; r < NT; ++r) s += tile[r * TPP + c];
+        if (diag) csum[c] = s;
+        else if (c < mc) S[bi * g.n + gc + c] = s;
+    }
+    if (diag) __syncthreads();
+    if (tid >= NT) {
+        const int r = tid - NT;
+        T s = 0;
+        const int cend = diag ? r : NT;   /* strict: the diagonal is in the column sum */
+        for (int c = 0; c < cend; ++c) s += tile[r * TPP + c];
+        if (r < mr) S[bj * g.n + gr + r] = diag ? s + csum[r] : s;
+    }
+}
+
+/* acc[3] (bits) = max_j sum_slot S[slot][j] */
+template <typename T>
+__global__ __launch_bounds__(256) void colsum_max_kernel(const T *S, LansfGeom g, unsigned long long *acc)
+{
+    T m = 0;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < g.n; j += (int64_t)gridDim.x * blockDim.x) {
+        /* 16 independent loads in flight per thread */
+        double part[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) part[u] = 0.0;
+        int b = 0;
+        for (; b + 16 <= g.nb; b += 16) {
+            T x[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) x[u] = S[(int64_t)(b + u) * g.n + j];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) part[u] += (double)x[u];
+        }
+        for (; b < g.nb; ++b) part[0] += (double)S[(int64_t)b * g.n + j];
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) s += part[u];
+        m = absmax(m, (T)s);
+    }
+    double unused = 0.0;
+    block_reduce(m, unused);
+    if (threadIdx.x == 0) atomicMax(acc + 3, key_of(m));
+}
+
+__global__ void zero_u64_kernel(unsigned long long *p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
+}
+
+LansfGeom lansf_geom(const RfpGeom &r)
+{
+    LansfGeom g;
+    g.v11 = r.v11; g.v21 = r.v21; g.v22 = r.v22;
+    g.n = r.n; g.n1 = r.n1; g.n2 = r.n2;
+    g.s1 = (int)((r.n1 + NT - 1) / NT);
+    g.nb = g.s1 + (int)((r.n2 + NT - 1) / NT);
+    return g;
+}
+
+template <typename T>
+hipError_t lansf(const RfpGeom &rg, int norm, const T *arf, void *ws, hipStream_t s)
+{
+    const LansfGeom g = lansf_geom(rg);
+    unsigned long long *acc = (unsigned long long *)ws;
+    hipLaunchKernelGGL(zero_u64_kernel, dim3(1), dim3(64), 0, s, acc, 4);
+    if (norm == 1) {
+        T *S = (T *)((char *)ws + 256);
+        const int64_t ntiles = (int64_t)g.nb * (g.nb + 1) / 2;
+        hipLaunchKernelGGL((colsum_tiles_kernel<T>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, S);
+        const unsigned blocks = (unsigned)std::min<int64_t>((g.n + 255) / 256, 2048);
+        hipLaunchKernelGGL((colsum_max_kernel<T>), dim3(blocks), dim3(256), 0, s, S, g, acc);
+    } else {
+        const int64_t nt = g.n * (g.n + 1) / 2;
+        const unsigned blocks = (unsigned)std::min<int64_t>((nt / Vec16<T>::N + 255) / 256 + 1, 4096);
+        hipLaunchKernelGGL((absmax_sumsq_kernel<T>), dim3(blocks), dim3(256), 0, s, arf, nt, acc);
+        if (norm == 2) {
+            const unsigned db = (unsigned)std::min<int64_t>((g.n + 255) / 256, 1024);
+            hipLaunchKernelGGL((diag_sumsq_kernel<T>), dim3(db), dim3(256), 0, s, arf, g, acc);
+        }
+    }
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t lansf_scratch_bytes(const RfpGeom &rg, int norm, size_t elem)
+{
+    const LansfGeom g = lansf_geom(rg);
+    return 256 + (norm == 1 ? (size_t)g.nb * (size_t)g.n * elem : 0);
+}
+
+hipError_t launch_lansf_f32(const RfpGeom &g, int norm, const float *arf, void *ws, hipStream_t s)
+{ return lansf<float>(g, norm, arf, ws, s); }
+hipError_t launch_lansf_f64(const RfpGeom &g, int norm, const double *arf, void *ws, hipStream_t s)
+{ return lansf<double>(g, norm, arf, ws, s); }

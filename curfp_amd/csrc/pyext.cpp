@@ -195,4 +195,16 @@ PYBIND11_MODULE(_curfp_C, m)
     m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
         check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
     });
+    m.def("slansf", [](Handle &h, int norm, int transr, int uplo, int n, int64_t arf) {
+        float r = 0.f;
+        check_status(curfpSlansf(h.get(), static_cast<curfpNormType_t>(norm), OPS(transr), FILL(uplo), n,
+                                 P<const float>(arf), &r));
+        return r;
+    });
+    m.def("dlansf", [](Handle &h, int norm, int transr, int uplo, int n, int64_t arf) {
+        double r = 0.0;
+        check_status(curfpDlansf(h.get(), static_cast<curfpNormType_t>(norm), OPS(transr), FILL(uplo), n,
+                                 P<const double>(arf), &r));
+        return r;
+    });
 }

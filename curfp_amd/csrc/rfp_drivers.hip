@@ -28,6 +28,7 @@
 #include <algorithm>
 #include <climits>
 #include <cmath>
+#include <cstring>
 #include <new>
 
 /* ------------------------------------------------------------------------ */
@@ -771,6 +772,60 @@ static curfpStatus_t add_diag_impl(curfpHandle_t h, curfpOperation_t transr, cur
 }
 
 /* ------------------------------------------------------------------------ */
+/* slansf / dlansf                                                           */
+/* ------------------------------------------------------------------------ */
+
+template <typename T>
+static curfpStatus_t lansf_impl(curfpHandle_t h, curfpNormType_t norm, curfpOperation_t transr, curfpFillMode_t uplo,
+                                int n, const T *arf, T *result)
+{
+    RFP_CHECK_HANDLE(h);
+    if (!result || n < 0) return CURFP_STATUS_INVALID_VALUE;
+    if ((norm != CURFP_NORM_MAX && norm != CURFP_NORM_ONE && norm != CURFP_NORM_FRO) ||
+        (transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    *result = T(0);
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!arf) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, lansf_scratch_bytes(g, (int)norm, sizeof(T)), &ws);
+    if (e != hipSuccess) return st_of(e);
+    const double nt = (double)n * (n + 1) / 2.0;
+    e = timed(h, CURFP_KERNEL_OTHER, 0.0, nt * sizeof(T), [&] {
+        if (sizeof(T) == 4) return launch_lansf_f32(g, (int)norm, (const float *)arf, ws, h->stream);
+        return launch_lansf_f64(g, (int)norm, (const double *)arf, ws, h->stream);
+    });
+    unsigned long long acc[4] = {0, 0, 0, 0};
+    if (e == hipSuccess) e = hipMemcpyAsync(acc, ws, sizeof(acc), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) return st_of(e);
+    auto value_of = [](unsigned long long k) -> T {
+        if (sizeof(T) == 4) {
+            const uint32_t b = (uint32_t)k;
+            float f;
+            memcpy(&f, &b, 4);
+            return (T)f;
+        }
+        double d;
+        memcpy(&d, &k, 8);
+        return (T)d;
+    };
+    if (norm == CURFP_NORM_MAX) {
+        *result = value_of(acc[0]);
+    } else if (norm == CURFP_NORM_ONE) {
+        *result = value_of(acc[3]);
+    } else {
+        double all, dg;
+        memcpy(&all, &acc[1], 8);
+        memcpy(&dg, &acc[2], 8);
+        *result = (T)sqrt(std::max(0.0, 2.0 * all - dg));
+    }
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
 /* FP64 generator residual                                                   */
 /* ------------------------------------------------------------------------ */
 
@@ -1062,5 +1117,13 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curf
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
                                   const double *value, double *arf)
 { return add_diag_impl<double>(h, transr, uplo, n, value, arf); }
+
+curfpStatus_t curfpSlansf(curfpHandle_t h, curfpNormType_t norm, curfpOperation_t transr, curfpFillMode_t uplo,
+                          int n, const float *arf, float *result)
+{ return lansf_impl<float>(h, norm, transr, uplo, n, arf, result); }
+
+curfpStatus_t curfpDlansf(curfpHandle_t h, curfpNormType_t norm, curfpOperation_t transr, curfpFillMode_t uplo,
+                          int n, const double *arf, double *result)
+{ return lansf_impl<double>(h, norm, transr, uplo, n, arf, result); }
 
 }  /* extern "C" */

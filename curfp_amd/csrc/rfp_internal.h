@@ -300,3 +300,14 @@ hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, d
 hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_t s);
 hipError_t launch_gram_norms_f32(int64_t k, const float *G, double *out, hipStream_t s);
 hipError_t launch_zero_f64(double *p, int n, hipStream_t s);
+
+/*
+ * Norms of the symmetric RFP matrix (lansf.hip): norm 0 = max |a_ij|,
+ * 1 = largest absolute column sum, 2 = Frobenius.  ws holds
+ * lansf_scratch_bytes; after the launches its first 32 bytes hold
+ * [0] max |a| (bit pattern of T, u64), [1] sum of squares (double),
+ * [2] sum of squared diagonal (double), [3] 1-norm (bit pattern of T, u64).
+ */
+size_t lansf_scratch_bytes(const RfpGeom &g, int norm, size_t elem);
+hipError_t launch_lansf_f32(const RfpGeom &g, int norm, const float *arf, void *ws, hipStream_t s);
+hipError_t launch_lansf_f64(const RfpGeom &g, int norm, const double *arf, void *ws, hipStream_t s);

@@ -50,6 +50,13 @@ typedef enum {
     CURFP_FILL_MODE_UPPER = 1
 } curfpFillMode_t;
 
+/* Norm type of curfpSlansf / curfpDlansf — reference include/curfp.h:46-53. */
+typedef enum {
+    CURFP_NORM_MAX = 0,   /* max |A(i,j)|                                  */
+    CURFP_NORM_ONE = 1,   /* largest absolute column sum (= infinity norm) */
+    CURFP_NORM_FRO = 2    /* Frobenius norm                                */
+} curfpNormType_t;
+
 /*
  * Arithmetic used by the contractions of ssfrk and of the spftrf updates
  * (TRSM / SYRK / GEMM).  No counterpart in the reference (which always runs
@@ -137,6 +144,15 @@ curfpStatus_t curfpSstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const float *value, float *arf);
+
+/* curfpSlansf — reference include/curfp.h:207-227, src/curfp_slansf.cu:273-435.
+ * Norm of the symmetric matrix stored in RFP (the matrix itself, not its
+ * Cholesky factor); *result is a host pointer, the call synchronizes the
+ * handle stream.  Unlike the reference (int-sized cuBLAS reductions,
+ * INVALID_VALUE once n(n+1)/2 > INT_MAX) every size is 64-bit. */
+curfpStatus_t curfpSlansf(curfpHandle_t handle, curfpNormType_t norm,
+                          curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                          const float *arf, float *result);
 
 /* curfpSsfrkBf16 — ssfrk with a BF16 A (raw 16-bit storage, same layout and
  * lda meaning as curfpSsfrk).  Products of BF16 values are exact in FP32, so
@@ -268,6 +284,12 @@ curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const double *value, double *arf);
+
+/* curfpDlansf — reference include/curfp.h:451-462, src/curfp_dlansf.cu:189.
+ * FP64 twin of curfpSlansf (64-bit sizes, host result, synchronizing). */
+curfpStatus_t curfpDlansf(curfpHandle_t handle, curfpNormType_t norm,
+                          curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                          const double *arf, double *result);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,11 @@ Functions and what pins each one (see tests/test_oracle.py):
         LAPACK info of crafted non-SPD matrices.
   pftrs  — reference src/curfp_spftrs.cpp:215-302 (4 TRSM + 2 GEMM); pinned
         by LAPACK ?pftrs golden vectors and the dense residual B - C X.
+  lansf  — reference src/curfp_slansf.cu:273-435 (max |a|, largest absolute
+        column sum, Frobenius norm of the symmetric matrix); pinned by
+        LAPACK ?lange on the dense matrix (an independent routine), the
+        closed forms of c J + d I and of the identity, and the n = 1 known
+        answer of python_tests/test_slansf.py:124-133.
 """
 from __future__ import annotations
 
@@ -42,7 +47,7 @@ from scipy.linalg import lapack as _lapack
 
 __all__ = [
     "views", "lower_index_map", "trttf", "tfttr", "diag_indices", "add_to_diagonal",
-    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "chain_flops",
+    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "lansf", "chain_flops",
 ]
 
 
@@ -316,6 +321,24 @@ def pftrs(n, arf, B, transr="T", uplo="U"):
         b1 = trsm(1.0, L11, b1, side=0, lower=1, trans_a=1)
     X = np.concatenate([b1, b2], axis=0).astype(dt)
     return X[:, 0] if squeeze else X
+
+
+def lansf(arf: np.ndarray, n: int, norm: str, transr: str = "T", uplo: str = "U") -> float:
+    """Norm of the symmetric matrix stored in RFP (reference
+    src/curfp_slansf.cu:273-435): 'M' max |a_ij|, '1' (= 'O' = 'I') the
+    largest absolute column sum, 'F' (= 'E') Frobenius; evaluated in float64
+    on the dense symmetric matrix."""
+    if n == 0:
+        return 0.0
+    S = np.abs(rfp_to_dense_sym(np.asarray(arf, dtype=np.float64), n, transr, uplo))
+    key = norm.upper()
+    if key in ("M", "MAX"):
+        return float(S.max())
+    if key in ("1", "O", "I"):
+        return float(S.sum(axis=0).max())
+    if key in ("F", "E", "FRO"):
+        return float(np.sqrt((S * S).sum()))
+    raise ValueError(f"unknown norm {norm!r}")
 
 
 def chain_flops(n: int, k: int, nrhs: int) -> dict:

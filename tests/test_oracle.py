@@ -196,3 +196,66 @@ def test_chain_flops_counts():
     assert abs(f["ssfrk"] - 4.3981e12) / 4.3981e12 < 1e-4
     assert abs(f["spftrf"] - 9.3827e13) / 9.3827e13 < 1e-4
     assert abs(f["spftrs"] - 5.4976e11) / 5.4976e11 < 1e-4
+
+
+# --------------------------------------------------------------------------- norms
+
+NORMS = ["M", "1", "F"]
+
+
+def _sym_with_skewed_triangles(n, seed):
+    """Symmetric S whose lower-triangle column sums differ from its row sums
+    (large entries low in the matrix), so a norm that forgets the mirrored
+    triangle is caught."""
+    rng = np.random.default_rng(seed)
+    L = np.tril(rng.standard_normal((n, n)) * (1.0 + np.arange(n))[:, None])
+    return L + np.tril(L, -1).T
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 8, 17, 64, 129, 257])
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("norm", NORMS)
+def test_lansf_matches_lapack_lange(n, t, u, norm):
+    from scipy.linalg import lapack
+
+    S = _sym_with_skewed_triangles(n, n)
+    arf = O.trttf(S, t, u)
+    ref = float(lapack.dlange(norm, S))
+    assert abs(O.lansf(arf, n, norm, t, u) - ref) <= 1e-13 * ref
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_lansf_closed_forms(t, u):
+    n, c, d = 37, 0.25, 3.0
+    S = np.full((n, n), c) + d * np.eye(n)
+    arf = O.trttf(S, t, u)
+    assert O.lansf(arf, n, "M", t, u) == c + d
+    assert abs(O.lansf(arf, n, "1", t, u) - (n * c + d)) < 1e-12
+    assert abs(O.lansf(arf, n, "F", t, u) - math.sqrt(n * (c + d) ** 2 + (n * n - n) * c * c)) < 1e-12
+    arf = O.trttf(np.eye(n), t, u)
+    assert (O.lansf(arf, n, "M", t, u), O.lansf(arf, n, "1", t, u)) == (1.0, 1.0)
+    assert abs(O.lansf(arf, n, "F", t, u) - math.sqrt(n)) < 1e-12
+
+
+def test_lansf_n1_and_n0_known_answers():
+    """python_tests/test_slansf.py:116-133: n = 0 gives 0; n = 1 gives |v|."""
+    for norm in ("M", "1", "O", "I", "F", "E", "fro", "max"):
+        assert O.lansf(np.zeros(0), 0, norm) == 0.0
+        assert abs(O.lansf(np.array([3.7]), 1, norm) - 3.7) < 1e-15
+
+
+def test_planted_norm_mistake_is_caught(monkeypatch):
+    """Column sums of the stored triangle alone (the mirrored half dropped)
+    must fail the LAPACK comparison."""
+    from scipy.linalg import lapack
+
+    n = 9
+    S = _sym_with_skewed_triangles(n, 4)
+    arf = O.trttf(S, "T", "U")
+    real = O.rfp_to_dense_sym
+
+    def lower_only(a, m, transr="T", uplo="U"):
+        return np.tril(real(a, m, transr, uplo))
+
+    monkeypatch.setattr(O, "rfp_to_dense_sym", lower_only)
+    assert abs(O.lansf(arf, n, "1", "T", "U") - lapack.dlange("1", S)) > 1e-3
