@@ -195,6 +195,49 @@ def cpu_baseline(n, k, nrhs, seed=1):
             "seconds": dt, "info": int(info)}
 
 
+def run_norms(curfp, torch, args, dev) -> int:
+    """slansf on a random n-order RFP matrix (uniform in [-1, 1)): per norm,
+    wall time per call (host result, one stream sync) and the library's
+    HIP-event time of its kernels.  Every norm reads the packed array once
+    (the 1-norm also writes and reads an (n/128) x n array of partial sums,
+    3 % more at n = 65536), so the roofline is HBM at n(n+1)/2 * 4 bytes per
+    call."""
+    n = args.n
+    nt = n * (n + 1) // 2
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    C = torch.rand(nt, device=dev, generator=g) * 2.0 - 1.0
+    steps = max(args.steps, 1)
+    out = {}
+    for norm in ("M", "1", "F"):
+        for _ in range(max(args.warmup, 1)):
+            curfp.slansf(C, norm)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            val = curfp.slansf(C, norm)
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        curfp.kernel_stats(reset=True)
+        curfp.set_profiling(True)
+        for _ in range(steps):
+            curfp.slansf(C, norm)
+        st = curfp.kernel_stats(reset=True)["other"]
+        curfp.set_profiling(False)
+        kms = st["ms"] / max(st["launches"], 1)
+        out[norm] = {"value": val, "wall_ms": wall, "kernel_ms": kms, "GBps": nt * 4 / (kms * 1e-3) / 1e9}
+        log(f"slansf {norm}: {val:.6g}, wall {wall:.3f} ms, kernels {kms:.3f} ms, {out[norm]['GBps']:.0f} GB/s")
+    worst = min(out.values(), key=lambda d: d["GBps"])
+    line = {"metric": "slansf_bandwidth", "value": worst["GBps"], "unit": "GB/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"slansf on a random RFP matrix, n={n}", "n": n, "bytes_per_call": nt * 4},
+            "norms": out,
+            "roofline": {"bound": "hbm", "achieved": worst["GBps"], "peak": 8000.0, "unit": "GB/s",
+                         "frac": worst["GBps"] / 8000.0, "traffic": None,
+                         "note": "slowest of the three norms (the 1-norm); kernel time from HIP events"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -205,7 +248,8 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir"], default="both")
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms"], default="both",
+                    help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
@@ -226,6 +270,8 @@ def main() -> int:
     import curfp_amd as curfp
 
     dev = torch.device("cuda:0")
+    if args.mode == "norms":
+        return run_norms(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
     fl = chain_flops(n, k, nrhs)
     log(f"inputs n={n} k={k} nrhs={nrhs} seed={args.seed}")
