@@ -61,6 +61,8 @@ try:
         dadd_to_diagonal as _dadd_diag_C,
         slansf as _slansf_C,
         dlansf as _dlansf_C,
+        ssfmm as _ssfmm_C,
+        dsfmm as _dsfmm_C,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(
@@ -513,6 +515,102 @@ def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R
 
 
 # ---------------------------------------------------------------------------
+# Symmetric RFP matrix times a dense block — curfp/__init__.py:1203-1360
+# ---------------------------------------------------------------------------
+_SIDE_MAP = {"L": SIDE_LEFT, "R": SIDE_RIGHT}
+
+
+def _side(s: str) -> int:
+    try:
+        return _SIDE_MAP[s.upper()]
+    except KeyError:
+        raise ValueError(f"side must be 'L' or 'R', got {s!r}") from None
+
+
+def ssfmm_raw(handle: Handle, transr: int, uplo: int, side: int, m: int, n: int, alpha: float,
+              arf: torch.Tensor, B: torch.Tensor, ldb: int, beta: float, C: torch.Tensor, ldc: int) -> None:
+    """Low-level ssfmm: integer enums, B and C column-major m x n with leading
+    dimensions ldb, ldc (side L: C = alpha A B + beta C, A m x m; side R:
+    C = alpha B A + beta C, A n x n)."""
+    _validate(arf, "arf")
+    _validate_strided(B, "B", torch.float32)
+    _validate_strided(C, "C", torch.float32)
+    _ssfmm_C(handle, transr, uplo, side, m, n, float(alpha), arf.data_ptr(), B.data_ptr(), ldb, float(beta),
+             C.data_ptr(), ldc)
+
+
+def dsfmm_raw(handle: Handle, transr: int, uplo: int, side: int, m: int, n: int, alpha: float,
+              arf: torch.Tensor, B: torch.Tensor, ldb: int, beta: float, C: torch.Tensor, ldc: int) -> None:
+    """FP64 twin of ssfmm_raw."""
+    _validate(arf, "arf", torch.float64)
+    _validate_strided(B, "B", torch.float64)
+    _validate_strided(C, "C", torch.float64)
+    _dsfmm_C(handle, transr, uplo, side, m, n, float(alpha), arf.data_ptr(), B.data_ptr(), ldb, float(beta),
+             C.data_ptr(), ldc)
+
+
+def _validate_strided(t: torch.Tensor, name: str, dtype) -> None:
+    """A dense operand addressed through an explicit leading dimension: CUDA and
+    of the right dtype (a transposed view is fine)."""
+    if not t.is_cuda or t.dtype != dtype:
+        raise ValueError(f"{name} must be a {dtype} CUDA tensor")
+
+
+def _sfmm(fn, dtype, arf, B, C, alpha, beta, transr, uplo, side, n_A):
+    if not (arf.is_cuda and arf.is_contiguous() and arf.dtype == dtype):
+        raise ValueError(f"arf must be a contiguous {dtype} CUDA tensor")
+    if B.ndim != 2 or not B.is_cuda or B.dtype != dtype:
+        raise ValueError(f"B must be a 2-D {dtype} CUDA tensor, got {tuple(B.shape)}")
+    m, n = B.shape
+    if n_A is None:
+        n_A = _n_from_rfp(arf.numel())
+    sd = side.upper()
+    if sd == "L" and n_A != m:
+        raise ValueError(f"side='L': A is {n_A}x{n_A} but B has {m} rows")
+    if sd == "R" and n_A != n:
+        raise ValueError(f"side='R': A is {n_A}x{n_A} but B has {n} cols")
+    if C is None:
+        C = torch.zeros(n, m, dtype=dtype, device=arf.device).t()     # column-major m x n
+    if C.shape != (m, n) or not C.is_cuda or C.dtype != dtype:
+        raise ValueError(f"C must be a {m}x{n} {dtype} CUDA tensor, got {tuple(C.shape)}")
+
+    def colmajor(t):
+        return t.stride(0) == 1 and (t.shape[1] <= 1 or t.stride(1) >= max(t.shape[0], 1))
+
+    def rowmajor(t):
+        return t.stride(1) == 1 and (t.shape[0] <= 1 or t.stride(0) >= max(t.shape[1], 1))
+
+    h = _get_handle()
+    if colmajor(B) and colmajor(C):
+        fn(h, _op(transr), _fill(uplo), _side(sd), m, n, float(alpha), arf.data_ptr(), B.data_ptr(),
+           max(B.stride(1), m, 1), float(beta), C.data_ptr(), max(C.stride(1), m, 1))
+    elif rowmajor(B) and rowmajor(C):
+        # row-major m x n is column-major n x m holding the transpose: C^T = alpha B^T A + beta C^T
+        # (or alpha A B^T + beta C^T), i.e. the other side
+        fn(h, _op(transr), _fill(uplo), _side("R" if sd == "L" else "L"), n, m, float(alpha), arf.data_ptr(),
+           B.data_ptr(), max(B.stride(0), n, 1), float(beta), C.data_ptr(), max(C.stride(0), n, 1))
+    else:
+        raise ValueError("B and C must both be column-major (F-contiguous) or both row-major")
+    return C
+
+
+def ssfmm(arf: torch.Tensor, B: torch.Tensor, C: torch.Tensor = None, *, alpha: float = 1.0, beta: float = 0.0,
+          transr: str = "T", uplo: str = "U", side: str = "L", n_A: int = None) -> torch.Tensor:
+    """Symmetric RFP matrix A times a dense block (curfp/__init__.py:1203-1302):
+    side='L': C := alpha*A@B + beta*C (A m x m); side='R': C := alpha*B@A + beta*C
+    (A n x n).  B, C float32 (m, n), both column-major or both row-major (the
+    row-major case runs as the transposed product); C is created (zeros,
+    column-major) when not given and returned."""
+    return _sfmm(_ssfmm_C, torch.float32, arf, B, C, alpha, beta, transr, uplo, side, n_A)
+
+
+def dsfmm(arf: torch.Tensor, B: torch.Tensor, C: torch.Tensor = None, *, alpha: float = 1.0, beta: float = 0.0,
+          transr: str = "T", uplo: str = "U", side: str = "L", n_A: int = None) -> torch.Tensor:
+    """FP64 twin of ssfmm."""
+    return _sfmm(_dsfmm_C, torch.float64, arf, B, C, alpha, beta, transr, uplo, side, n_A)
+
+
+# ---------------------------------------------------------------------------
 # Norms of a symmetric RFP matrix — curfp/__init__.py:784-841 (slansf),
 # :1700-1720 (dlansf)
 # ---------------------------------------------------------------------------
@@ -561,8 +659,8 @@ def _not_built(name):
     return fn
 
 
-for _name in ("spftri", "ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "ssfmm", "spftrf_rcond",
-              "dpftri", "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dsfmm", "dpftrf_rcond"):
+for _name in ("spftri", "ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "spftrf_rcond",
+              "dpftri", "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dpftrf_rcond"):
     globals()[_name] = _not_built(_name)
     globals()[_name + "_raw"] = _not_built(_name + "_raw")
 del _name

@@ -195,6 +195,16 @@ PYBIND11_MODULE(_curfp_C, m)
     m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
         check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
     });
+    m.def("ssfmm", [](Handle &h, int transr, int uplo, int side, int m, int n, float alpha, int64_t arf, int64_t B,
+                      int ldb, float beta, int64_t C, int ldc) {
+        check_status(curfpSsfmm(h.get(), OPS(transr), FILL(uplo), static_cast<curfpSideMode_t>(side), m, n, &alpha,
+                                P<const float>(arf), P<const float>(B), ldb, &beta, P<float>(C), ldc));
+    });
+    m.def("dsfmm", [](Handle &h, int transr, int uplo, int side, int m, int n, double alpha, int64_t arf, int64_t B,
+                      int ldb, double beta, int64_t C, int ldc) {
+        check_status(curfpDsfmm(h.get(), OPS(transr), FILL(uplo), static_cast<curfpSideMode_t>(side), m, n, &alpha,
+                                P<const double>(arf), P<const double>(B), ldb, &beta, P<double>(C), ldc));
+    });
     m.def("slansf", [](Handle &h, int norm, int transr, int uplo, int n, int64_t arf) {
         float r = 0.f;
         check_status(curfpSlansf(h.get(), static_cast<curfpNormType_t>(norm), OPS(transr), FILL(uplo), n,

@@ -826,6 +826,43 @@ static curfpStatus_t lansf_impl(curfpHandle_t h, curfpNormType_t norm, curfpOper
 }
 
 /* ------------------------------------------------------------------------ */
+/* ssfmm / dsfmm                                                             */
+/* ------------------------------------------------------------------------ */
+
+/* Reference src/curfp_ssfmm.cpp:107-306: side L  C := alpha A B + beta C (A m x m),
+ * side R  C := alpha B A + beta C (A n x n); B and C column-major m x n.  Side R
+ * runs as C^T = alpha A B^T + beta C^T on transposed views. */
+template <typename T>
+static curfpStatus_t sfmm_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpSideMode_t side,
+                               int m, int n, const T *alpha, const T *arf, const T *B, int ldb, const T *beta, T *C,
+                               int ldc)
+{
+    RFP_CHECK_HANDLE(h);
+    if (m < 0 || n < 0 || ldb < 1 || ldc < 1) return CURFP_STATUS_INVALID_VALUE;
+    if (!alpha || !beta) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER) ||
+        (side != CURFP_SIDE_LEFT && side != CURFP_SIDE_RIGHT))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (m == 0 || n == 0) return CURFP_STATUS_SUCCESS;
+    if (ldb < m || ldc < m || !B || !C || (*alpha != T(0) && !arf)) return CURFP_STATUS_INVALID_VALUE;
+    const bool left = side == CURFP_SIDE_LEFT;
+    const int64_t nA = left ? m : n, N = left ? n : m;
+    const RfpGeom g = rfp_geom(nA, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const RfpView yv = left ? RfpView{0, 1, ldb} : RfpView{0, ldb, 1};
+    const RfpView cv = left ? RfpView{0, 1, ldc} : RfpView{0, ldc, 1};
+    const double fl = 2.0 * (double)nA * (double)nA * (double)N;
+    const double by = (double)nA * (nA + 1) / 2.0 * sizeof(T) + 3.0 * (double)nA * N * sizeof(T);
+    return st_of(timed(h, Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64 : CURFP_KERNEL_GEMM_F32, fl, by, [&] {
+        if (Prec<T>::f64)
+            return launch_sfmm_f64((const double *)arf, g, (const double *)B, yv, (double *)C, cv, N,
+                                   (double)*alpha, (double)*beta, h->stream);
+        return launch_sfmm_f32((const float *)arf, g, (const float *)B, yv, (float *)C, cv, N, (float)*alpha,
+                               (float)*beta, h->stream);
+    }));
+}
+
+/* ------------------------------------------------------------------------ */
 /* FP64 generator residual                                                   */
 /* ------------------------------------------------------------------------ */
 
@@ -1117,6 +1154,16 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curf
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
                                   const double *value, double *arf)
 { return add_diag_impl<double>(h, transr, uplo, n, value, arf); }
+
+curfpStatus_t curfpSsfmm(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpSideMode_t side,
+                         int m, int n, const float *alpha, const float *arf, const float *B, int ldb,
+                         const float *beta, float *C, int ldc)
+{ return sfmm_impl<float>(h, transr, uplo, side, m, n, alpha, arf, B, ldb, beta, C, ldc); }
+
+curfpStatus_t curfpDsfmm(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpSideMode_t side,
+                         int m, int n, const double *alpha, const double *arf, const double *B, int ldb,
+                         const double *beta, double *C, int ldc)
+{ return sfmm_impl<double>(h, transr, uplo, side, m, n, alpha, arf, B, ldb, beta, C, ldc); }
 
 curfpStatus_t curfpSlansf(curfpHandle_t h, curfpNormType_t norm, curfpOperation_t transr, curfpFillMode_t uplo,
                           int n, const float *arf, float *result)

@@ -273,6 +273,14 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
                                   RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
                                   unsigned long long epoch, hipStream_t s);
+/* C := alpha * A * Y + beta * C (trsv_block.hip) with A the n x n symmetric
+ * matrix stored in RFP at A (geometry g), Y an n x N view and C an n x N view
+ * (element (i, c) at off + i*rs + c*cs).  Y and C must not overlap.  FP32
+ * on MFMA; FP64 on VALU FMAs. */
+hipError_t launch_sfmm_f32(const float *A, const RfpGeom &g, const float *B, RfpView bv, float *C, RfpView cv,
+                           int64_t N, float alpha, float beta, hipStream_t s);
+hipError_t launch_sfmm_f64(const double *A, const RfpGeom &g, const double *B, RfpView bv, double *C, RfpView cv,
+                           int64_t N, double alpha, double beta, hipStream_t s);
 /* The sweep's products with the diagonal inverses, 4 * (blocks) 128 x 128
  * matrices at pm (written by this launch, read by both sweeps). */
 hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *inv, float *pm, hipStream_t s);

@@ -21,6 +21,12 @@
  * v_mfma_f32_16x16x4_f32 sequence over a 128 x 128 operand staged in LDS.
  *
  * The leaf inverses come from the batched inversion kernel (trtri_tiles).
+ *
+ * sfmm_kernel (ssfmm, FP32) reuses the sweep's block geometry and products:
+ * C = alpha * A * Y + beta * C with A the symmetric RFP matrix, one
+ * workgroup per 128-row block of A and group of <= 64 columns of Y, summing
+ * A(b, c) Y_c over every block c with no dependencies between workgroups.
+ * sfmm64_kernel is its FP64 twin (VALU FMAs, 64 x 64 output tiles).
  */
 #include "rfp_internal.h"
 #include "tile_stage.h"
@@ -788,6 +794,188 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* ssfmm / dsfmm: symmetric RFP matrix times a dense block                   */
+/* ------------------------------------------------------------------------ */
+
+struct SfmmArgs {
+    SweepArgs s;        /* geometry, A at s.L, Y at s.B / s.bv, its column count in s.nrhs */
+    float *C;
+    RfpView cv;         /* C(i, c) = C[cv.off + i*cv.rs + c*cv.cs] */
+    float alpha, beta;
+};
+
+/*
+ * C(rows of block b, columns of group y) = alpha * sum_c A(b, c) Y_c + beta C.
+ * Tile (b, c) of the symmetric A is the stored tile for b > c, the stored
+ * tile (c, b) read transposed for b < c, and for b == c the stored triangle
+ * read with mirrored indices (so no entry outside the triangle is read).  The
+ * next off-diagonal tile is staged in registers while the current products
+ * run; the next Y block is loaded as soon as the current one is in LDS.
+ */
+template <int CW, bool YCM>
+__global__ __launch_bounds__(256) void sfmm_kernel(SfmmArgs p)
+{
+    constexpr int CT = CW / 16;
+    __shared__ __attribute__((aligned(16))) float As[LF * SP];
+    __shared__ __attribute__((aligned(16))) float Ys[ys_elems(CW)];
+    const SweepArgs &a = p.s;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int b = blockIdx.x;
+    const int col0 = blockIdx.y * CW, cols = min(CW, a.nrhs - col0);
+    int64_t grow, lrow;
+    int rows;
+    sweep_block(a, b, grow, lrow, rows);
+    auto tile_of = [&](int c) {
+        int64_t gc, lc;
+        int mc;
+        sweep_block(a, c, gc, lc, mc);
+        if (b > c) {
+            const RfpView v = sweep_tile(a, b, c);
+            return Blk{a.L, v.off, v.rs, v.cs, rows, mc};
+        }
+        const RfpView v = sweep_tile(a, c, b);      /* A(b-row i, c-col k) = stored (c-row k, b-col i) */
+        return Blk{a.L, v.off, v.cs, v.rs, rows, mc};
+    };
+    auto load_y = [&](YStage<CW, YCM> &y, int c) {
+        int64_t gc, lc;
+        int mc;
+        sweep_block(a, c, gc, lc, mc);
+        y.load(a, gc, mc, col0, cols, tid);
+    };
+
+    f32x4v acc[2][CT];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) acc[t][ct][r] = 0.f;
+
+    /* alpha == 0: A and Y are not read, C := beta C */
+    const int nsteps = p.alpha == 0.f ? 0 : a.nb;
+    Staged st;
+    bool kcn = false;
+    {
+        const int c0 = b == 0 ? 1 : 0;
+        if (c0 < nsteps) {
+            const Blk bk = tile_of(c0);
+            kcn = bk.cs == 1;
+            st.load(bk, tid);
+        }
+    }
+    YStage<CW, YCM> yv;
+    if (nsteps > 0) load_y(yv, 0);
+    for (int c = 0; c < nsteps; ++c) {
+        lds_barrier();     /* the previous products are done with As and Ys */
+        bool kc = true;
+        if (c == b) {
+            const RfpView v = sweep_tile(a, b, b);
+            for (int e = 0; e < LF * LF / 256; ++e) {
+                const int idx = tid + 256 * e;
+                const int i = idx / LF, k = idx % LF;
+                As[i * SP + k] = (i < rows && k < rows) ? a.L[v.at(max(i, k), min(i, k))] : 0.f;
+            }
+        } else {
+            st.store(As, SP, tid);
+            kc = kcn;
+            const int cn = c + 1 == b ? c + 2 : c + 1;
+            if (cn < nsteps) {
+                const Blk bk = tile_of(cn);
+                kcn = bk.cs == 1;
+                st.load(bk, tid);
+            }
+        }
+        yv.store_lds(Ys, tid);
+        lds_barrier();
+        if (c + 1 < nsteps) load_y(yv, c + 1);
+        if (kc) mma_sweep<CT, true, 0>(acc, As, Ys, wave, lane);
+        else    mma_sweep<CT, false, 0>(acc, As, Ys, wave, lane);
+    }
+    const bool use_beta = p.beta != 0.f;
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int ct = 0; ct < CT; ++ct)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int row = sweep_row_tile(wave, t) * 16 + 4 * (lane >> 4) + r, col = ct * 16 + (lane & 15);
+                if (row < rows && col < cols) {
+                    float *q = p.C + p.cv.off + (grow + row) * p.cv.rs + (int64_t)(col0 + col) * p.cv.cs;
+                    const float v = p.alpha * acc[t][ct][r];
+                    *q = use_beta ? v + p.beta * *q : v;
+                }
+            }
+}
+
+struct Sfmm64Args {
+    const double *A;
+    RfpView v11, v21, v22;
+    int64_t n, n1;
+    const double *B;
+    RfpView bv;
+    double *C;
+    RfpView cv;
+    int64_t N;
+    double alpha, beta;
+};
+
+/* element (r, c) of the symmetric matrix from its stored lower triangle */
+__device__ __forceinline__ double sym64(const Sfmm64Args &p, int64_t r, int64_t c)
+{
+    const int64_t i = r > c ? r : c, j = r > c ? c : r;
+    if (i < p.n1) return p.A[p.v11.at(i, j)];
+    if (j < p.n1) return p.A[p.v21.at(i - p.n1, j)];
+    return p.A[p.v22.at(i - p.n1, j - p.n1)];
+}
+
+/* FP64: 64 x 64 tile of C per workgroup, 4 x 4 outputs per thread, k in
+ * slices of 16 through LDS */
+__global__ __launch_bounds__(256) void sfmm64_kernel(Sfmm64Args p)
+{
+    constexpr int T = 64, KS = 16;
+    __shared__ double As[KS][T + 1];
+    __shared__ double Bs[KS][T + 1];
+    const int tid = threadIdx.x, tr = tid / 16, tc = tid % 16;
+    const int64_t r0 = (int64_t)blockIdx.x * T, c0 = (int64_t)blockIdx.y * T;
+    double acc[4][4] = {};
+    const int64_t kend = p.alpha == 0.0 ? 0 : p.n;   /* alpha == 0: A and Y are not read */
+    for (int64_t k0 = 0; k0 < kend; k0 += KS) {
+        for (int e = tid; e < KS * T; e += 256) {
+            const int kk = e / T, x = e % T;
+            const int64_t gk = k0 + kk;
+            As[kk][x] = (r0 + x < p.n && gk < p.n) ? sym64(p, r0 + x, gk) : 0.0;
+            Bs[kk][x] = (gk < p.n && c0 + x < p.N) ? p.B[p.bv.off + gk * p.bv.rs + (c0 + x) * p.bv.cs] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KS; ++kk) {
+            double av[4], bv[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                av[u] = As[kk][tr + 16 * u];
+                bv[u] = Bs[kk][tc + 16 * u];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int w = 0; w < 4; ++w) acc[u][w] = fma(av[u], bv[w], acc[u][w]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const int64_t r = r0 + tr + 16 * u, c = c0 + tc + 16 * w;
+            if (r < p.n && c < p.N) {
+                double *q = p.C + p.cv.off + r * p.cv.rs + c * p.cv.cs;
+                const double v = p.alpha * acc[u][w];
+                *q = p.beta != 0.0 ? v + p.beta * *q : v;
+            }
+        }
+}
+
 }  // namespace
 
 int block_solve_rows() { return MAXQ * LF; }
@@ -859,5 +1047,48 @@ hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float 
     else SWEEP_LAUNCH_CW(64);
 #undef SWEEP_LAUNCH_CW
 #undef SWEEP_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_sfmm_f32(const float *A, const RfpGeom &g, const float *B, RfpView bv, float *C, RfpView cv,
+                           int64_t N, float alpha, float beta, hipStream_t s)
+{
+    if (g.n <= 0 || N <= 0) return hipSuccess;
+    const int cw = N <= 16 ? 16 : N <= 32 ? 32 : 64;
+    const int64_t groups = (N + cw - 1) / cw;
+    if (groups > 65535 || N > INT32_MAX) return hipErrorInvalidValue;
+    SfmmArgs p{};
+    p.s = sweep_args(A, g, nullptr, nullptr);
+    p.s.B = const_cast<float *>(B);
+    p.s.bv = bv;
+    p.s.nrhs = (int)N;
+    p.C = C;
+    p.cv = cv;
+    p.alpha = alpha;
+    p.beta = beta;
+    const dim3 grid((unsigned)p.s.nb, (unsigned)groups);
+    const bool ycm = bv.rs == 1;
+#define SFMM_LAUNCH(CW_)                                                                          \
+    do {                                                                                          \
+        if (ycm) hipLaunchKernelGGL((sfmm_kernel<CW_, true>), grid, dim3(256), 0, s, p);          \
+        else     hipLaunchKernelGGL((sfmm_kernel<CW_, false>), grid, dim3(256), 0, s, p);         \
+    } while (0)
+    if (cw == 16) SFMM_LAUNCH(16);
+    else if (cw == 32) SFMM_LAUNCH(32);
+    else SFMM_LAUNCH(64);
+#undef SFMM_LAUNCH
+    return hipGetLastError();
+}
+
+hipError_t launch_sfmm_f64(const double *A, const RfpGeom &g, const double *B, RfpView bv, double *C, RfpView cv,
+                           int64_t N, double alpha, double beta, hipStream_t s)
+{
+    if (g.n <= 0 || N <= 0) return hipSuccess;
+    const int64_t gx = (g.n + 63) / 64, gy = (N + 63) / 64;
+    if (gy > 65535) return hipErrorInvalidValue;
+    Sfmm64Args p;
+    p.A = A; p.v11 = g.v11; p.v21 = g.v21; p.v22 = g.v22; p.n = g.n; p.n1 = g.n1;
+    p.B = B; p.bv = bv; p.C = C; p.cv = cv; p.N = N; p.alpha = alpha; p.beta = beta;
+    hipLaunchKernelGGL(sfmm64_kernel, dim3((unsigned)gx, (unsigned)gy), dim3(256), 0, s, p);
     return hipGetLastError();
 }

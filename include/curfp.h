@@ -50,6 +50,12 @@ typedef enum {
     CURFP_FILL_MODE_UPPER = 1
 } curfpFillMode_t;
 
+/* Side of curfpSsfmm / curfpDsfmm — reference include/curfp.h:38-44. */
+typedef enum {
+    CURFP_SIDE_LEFT  = 0,   /* C = alpha * A * B + beta * C */
+    CURFP_SIDE_RIGHT = 1    /* C = alpha * B * A + beta * C */
+} curfpSideMode_t;
+
 /* Norm type of curfpSlansf / curfpDlansf — reference include/curfp.h:46-53. */
 typedef enum {
     CURFP_NORM_MAX = 0,   /* max |A(i,j)|                                  */
@@ -144,6 +150,16 @@ curfpStatus_t curfpSstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const float *value, float *arf);
+
+/* curfpSsfmm — reference include/curfp.h:322-348, src/curfp_ssfmm.cpp:107-306.
+ * side L: C := alpha*A*B + beta*C (A m x m); side R: C := alpha*B*A + beta*C
+ * (A n x n); A symmetric in RFP, B and C column-major m x n (ldb, ldc >= m),
+ * B and C not overlapping.  alpha == 0: A and B are not read.  beta == 0: C is
+ * not read.  One launch on FP32 MFMA (no cuBLAS SYMM + GEMM split). */
+curfpStatus_t curfpSsfmm(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, curfpSideMode_t side, int m, int n,
+                         const float *alpha, const float *arf, const float *B, int ldb,
+                         const float *beta, float *C, int ldc);
 
 /* curfpSlansf — reference include/curfp.h:207-227, src/curfp_slansf.cu:273-435.
  * Norm of the symmetric matrix stored in RFP (the matrix itself, not its
@@ -284,6 +300,13 @@ curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const double *value, double *arf);
+
+/* curfpDsfmm — reference include/curfp.h:528-544, src/curfp_dsfmm.cpp:78-219.  FP64 twin of
+ * curfpSsfmm. */
+curfpStatus_t curfpDsfmm(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, curfpSideMode_t side, int m, int n,
+                         const double *alpha, const double *arf, const double *B, int ldb,
+                         const double *beta, double *C, int ldc);
 
 /* curfpDlansf — reference include/curfp.h:451-462, src/curfp_dlansf.cu:189.
  * FP64 twin of curfpSlansf (64-bit sizes, host result, synchronizing). */

@@ -33,6 +33,11 @@ Functions and what pins each one (see tests/test_oracle.py):
         LAPACK info of crafted non-SPD matrices.
   pftrs  — reference src/curfp_spftrs.cpp:215-302 (4 TRSM + 2 GEMM); pinned
         by LAPACK ?pftrs golden vectors and the dense residual B - C X.
+  sfmm   — reference src/curfp_ssfmm.cpp:107-306 (side L: alpha S B + beta C,
+        side R: alpha B S + beta C with S the symmetric matrix); pinned against
+        the dense product of an independently built symmetric matrix, the
+        known answer of S = I, alpha = 0 / beta = 0 cases, and a planted
+        one-triangle mistake.
   lansf  — reference src/curfp_slansf.cu:273-435 (max |a|, largest absolute
         column sum, Frobenius norm of the symmetric matrix); pinned by
         LAPACK ?lange on the dense matrix (an independent routine), the
@@ -47,7 +52,7 @@ from scipy.linalg import lapack as _lapack
 
 __all__ = [
     "views", "lower_index_map", "trttf", "tfttr", "diag_indices", "add_to_diagonal",
-    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "lansf", "chain_flops",
+    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "lansf", "sfmm", "chain_flops",
 ]
 
 
@@ -339,6 +344,20 @@ def lansf(arf: np.ndarray, n: int, norm: str, transr: str = "T", uplo: str = "U"
     if key in ("F", "E", "FRO"):
         return float(np.sqrt((S * S).sum()))
     raise ValueError(f"unknown norm {norm!r}")
+
+
+def sfmm(arf: np.ndarray, nA: int, B: np.ndarray, C: np.ndarray, alpha: float, beta: float, transr: str = "T",
+         uplo: str = "U", side: str = "L") -> np.ndarray:
+    """Symmetric RFP matrix times a dense block (reference
+    src/curfp_ssfmm.cpp:107-306), float64: side 'L' alpha S B + beta C,
+    side 'R' alpha B S + beta C.  alpha == 0 does not read S or B; beta == 0
+    does not read C (BLAS conventions)."""
+    B = np.asarray(B, dtype=np.float64)
+    out = np.zeros(B.shape) if beta == 0 else beta * np.asarray(C, dtype=np.float64)
+    if alpha == 0 or nA == 0:
+        return out
+    S = rfp_to_dense_sym(np.asarray(arf, dtype=np.float64), nA, transr, uplo)
+    return out + alpha * (S @ B if side.upper() == "L" else B @ S)
 
 
 def chain_flops(n: int, k: int, nrhs: int) -> dict:

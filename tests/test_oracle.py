@@ -259,3 +259,42 @@ def test_planted_norm_mistake_is_caught(monkeypatch):
 
     monkeypatch.setattr(O, "rfp_to_dense_sym", lower_only)
     assert abs(O.lansf(arf, n, "1", "T", "U") - lapack.dlange("1", S)) > 1e-3
+
+
+# --------------------------------------------------------------------------- symmetric multiply
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 16, 33, 64, 129])
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("side", ["L", "R"])
+def test_sfmm_matches_dense_product(n, t, u, side):
+    rng = np.random.default_rng(7 * n + len(side))
+    S = _sym_with_skewed_triangles(n, n + 1)
+    arf = O.trttf(S, t, u)
+    nrhs = 5
+    B = rng.standard_normal((n, nrhs) if side == "L" else (nrhs, n))
+    C = rng.standard_normal(B.shape)
+    ref = 1.5 * (S @ B if side == "L" else B @ S) - 0.5 * C
+    assert np.allclose(O.sfmm(arf, n, B, C, 1.5, -0.5, t, u, side), ref, rtol=1e-13, atol=1e-12)
+
+
+def test_sfmm_known_answers():
+    n = 6
+    arf = O.trttf(np.eye(n), "N", "L")
+    B = np.arange(12.0).reshape(n, 2)
+    assert np.array_equal(O.sfmm(arf, n, B, None, 2.0, 0.0, "N", "L", "L"), 2.0 * B)
+    C = np.full((n, 2), np.nan)
+    assert np.array_equal(O.sfmm(arf, n, B, C, 1.0, 0.0, "N", "L", "L"), B)          # beta = 0: C not read
+    Bn = np.full((n, 2), np.nan)
+    C = np.ones((n, 2))
+    assert np.array_equal(O.sfmm(arf * np.nan, n, Bn, C, 0.0, 3.0, "N", "L", "L"), 3.0 * C)   # alpha = 0
+
+
+def test_planted_sfmm_mistake_is_caught(monkeypatch):
+    """Multiplying by the stored triangle alone must fail the dense check."""
+    n = 9
+    S = _sym_with_skewed_triangles(n, 2)
+    arf = O.trttf(S, "T", "U")
+    B = np.ones((n, 3))
+    real = O.rfp_to_dense_sym
+    monkeypatch.setattr(O, "rfp_to_dense_sym", lambda a, m, transr="T", uplo="U": np.tril(real(a, m, transr, uplo)))
+    assert not np.allclose(O.sfmm(arf, n, B, None, 1.0, 0.0, "T", "U", "L"), S @ B)
