@@ -238,6 +238,48 @@ def run_norms(curfp, torch, args, dev) -> int:
     return 0
 
 
+def run_sfmm(curfp, torch, args, dev) -> int:
+    """ssfmm C := A B (side L) with A a random n-order RFP matrix and B a
+    column-major n x nrhs block: 2 n^2 nrhs flops per call on the FP32 matrix
+    units; wall time per call and the library's HIP-event kernel time."""
+    n, nrhs = args.n, args.nrhs
+    nt = n * (n + 1) // 2
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    arf = torch.rand(nt, device=dev, generator=g) * 2.0 - 1.0
+    B = (torch.rand(nrhs, n, device=dev, generator=g) * 2.0 - 1.0).t()
+    C = torch.zeros(nrhs, n, device=dev).t()
+    steps = max(args.steps, 1)
+    for _ in range(max(args.warmup, 1)):
+        curfp.ssfmm(arf, B, C)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        curfp.ssfmm(arf, B, C)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    curfp.kernel_stats(reset=True)
+    curfp.set_profiling(True)
+    for _ in range(steps):
+        curfp.ssfmm(arf, B, C)
+    st = curfp.kernel_stats(reset=True)["gemm_f32"]
+    curfp.set_profiling(False)
+    kms = st["ms"] / max(st["launches"], 1)
+    fl = 2.0 * n * n * nrhs
+    tf = fl / (kms * 1e-3) / 1e12
+    log(f"ssfmm n={n} nrhs={nrhs}: wall {wall:.3f} ms, kernel {kms:.3f} ms, {tf:.1f} TFLOP/s")
+    line = {"metric": "ssfmm_tflops", "value": tf, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": wall, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"ssfmm side L, n={n}, nrhs={nrhs}", "n": n, "nrhs": nrhs},
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+                         "traffic": None, "note": "kernel time from HIP events; every stored 128 x 128 tile "
+                                                  "of A is read twice per column group, as A(b, c) and as "
+                                                  "A(c, b)^T"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,8 +290,9 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms"], default="both",
-                    help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead")
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm"], default="both",
+                    help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
+                         "sfmm: time ssfmm (side L, nrhs columns)")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
@@ -272,6 +315,8 @@ def main() -> int:
     dev = torch.device("cuda:0")
     if args.mode == "norms":
         return run_norms(curfp, torch, args, dev)
+    if args.mode == "sfmm":
+        return run_sfmm(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
     fl = chain_flops(n, k, nrhs)
     log(f"inputs n={n} k={k} nrhs={nrhs} seed={args.seed}")
