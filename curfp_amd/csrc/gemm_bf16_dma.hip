@@ -125,9 +125,10 @@ __device__ __forceinline__ int frag_off(int t, int lane)
  * operand fragments double-buffered in registers (step t + 1's read two at a
  * time between step t's groups of F MFMAs), and during step 0 the loads of
  * the next slice (into Na, Nb) issued the same way.  The issue order is fixed
- * with scheduling barriers.
+ * with scheduling barriers.  SWAP: the MFMA operands exchanged, so acc[i][j]
+ * holds the transpose of C's block (i, j).
  */
-template <int BT>
+template <int BT, bool SWAP>
 __device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F], const unsigned char *As,
                                           const unsigned char *Bs, unsigned char *Na, unsigned char *Nb,
                                           const SliceDma &da, const SliceDma &db, __amdgpu_buffer_rsrc_t rsa,
@@ -166,13 +167,14 @@ __device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int j = 0; j < F; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[cur].a[i], f[cur].b[j], acc[i][j], 0, 0, 0);
+                acc[i][j] = SWAP ? __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[cur].b[j], f[cur].a[i], acc[i][j], 0, 0, 0)
+                                 : __builtin_amdgcn_mfma_f32_32x32x16_bf16(f[cur].a[i], f[cur].b[j], acc[i][j], 0, 0, 0);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
 
-template <int BT>
+template <int BT, bool SWAP>
 __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
 {
     constexpr int F = Tile<BT>::F, SLICE = Tile<BT>::SLICE;
@@ -230,10 +232,10 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
     for (int s = 0; s < nk; s += 2) {
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_s_barrier();
-        mma_slice<BT>(acc, sA0, sB0, sA1, sB1, da, db, rsa, rsb, lda2, ldb2, (s + 1) * BK, K, wm, wn, wave, lane);
+        mma_slice<BT, SWAP>(acc, sA0, sB0, sA1, sB1, da, db, rsa, rsb, lda2, ldb2, (s + 1) * BK, K, wm, wn, wave, lane);
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_s_barrier();
-        mma_slice<BT>(acc, sA1, sB1, sA0, sB0, da, db, rsa, rsb, lda2, ldb2, (s + 2) * BK, K, wm, wn, wave, lane);
+        mma_slice<BT, SWAP>(acc, sA1, sB1, sA0, sB0, da, db, rsa, rsb, lda2, ldb2, (s + 2) * BK, K, wm, wn, wave, lane);
     }
     /* drain the zero-fill loads of the slice beyond K */
     __builtin_amdgcn_s_waitcnt(0);
@@ -243,6 +245,44 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
      * (rows contiguous) each such run of 4 rows is one 16-byte store (and
      * load when beta != 0) instead of four scattered ones. */
     const bool use_beta = p.beta != 0.f;
+    if (SWAP) {
+        /* acc[i][j] is C's block (i, j) transposed: a lane holds row
+         * 32 i + lane % 32 and, for each g, 4 consecutive columns 8 g + 4 (lane / 32)
+         * .. + 3 — one 16-byte store (and load when beta != 0) into a row view
+         * (columns contiguous, c_cs == 1) */
+#pragma unroll
+        for (int i = 0; i < F; ++i)
+#pragma unroll
+            for (int j = 0; j < F; ++j) {
+                const int64_t gi = m0 + wm * (BT / 2) + i * 32 + (lane & 31);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int64_t gj0 = n0 + wn * (BT / 2) + j * 32 + 8 * g + 4 * (lane >> 5);
+                    if (gi < p.M && gj0 + 3 < p.N && (!p.lower || gi >= gj0 + 3)) {
+                        float *cp = p.c + p.c_off + gi * p.c_rs + gj0;
+                        f32x4u v = {p.alpha * acc[i][j][4 * g], p.alpha * acc[i][j][4 * g + 1],
+                                    p.alpha * acc[i][j][4 * g + 2], p.alpha * acc[i][j][4 * g + 3]};
+                        if (use_beta) {
+                            const f32x4u o = *(const f32x4u *)cp;
+                            v = v + p.beta * o;
+                        }
+                        *(f32x4u *)cp = v;
+                        continue;
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int64_t gj = gj0 + e;
+                        if (gi < p.M && gj < p.N && (!p.lower || gi >= gj)) {
+                            float *cp = p.c + p.c_off + gi * p.c_rs + gj;
+                            float val = p.alpha * acc[i][j][4 * g + e];
+                            if (use_beta) val += p.beta * (*cp);
+                            *cp = val;
+                        }
+                    }
+                }
+            }
+        return;
+    }
     const bool rows_contig = p.c_rs == 1;
 #pragma unroll
     for (int i = 0; i < F; ++i)
@@ -310,7 +350,19 @@ hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s)
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
     if (p.ntiles <= 0) return hipSuccess;
     if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    if (bt == 256) hipLaunchKernelGGL(gemm_bf16_dma_kernel<256>, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
-    else           hipLaunchKernelGGL(gemm_bf16_dma_kernel<128>, dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    /* C a row view (columns contiguous) read and written (beta != 0, the
+     * factorisation's updates): the transposed accumulator layout gives the
+     * epilogue 16-byte loads and stores along rows.  With beta == 0 the
+     * plain layout's 4-byte stores already cover whole 128-byte lines of a
+     * row per instruction and measured faster (ssfrk). */
+    const bool swap = g.vc.cs == 1 && g.vc.rs != 1 && g.beta != 0.0;
+    const dim3 grid((unsigned)p.ntiles);
+    if (bt == 256) {
+        if (swap) hipLaunchKernelGGL((gemm_bf16_dma_kernel<256, true>), grid, dim3(256), 0, s, p);
+        else      hipLaunchKernelGGL((gemm_bf16_dma_kernel<256, false>), grid, dim3(256), 0, s, p);
+    } else {
+        if (swap) hipLaunchKernelGGL((gemm_bf16_dma_kernel<128, true>), grid, dim3(256), 0, s, p);
+        else      hipLaunchKernelGGL((gemm_bf16_dma_kernel<128, false>), grid, dim3(256), 0, s, p);
+    }
     return hipGetLastError();
 }
