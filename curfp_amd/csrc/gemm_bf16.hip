@@ -312,7 +312,9 @@ struct RowStage {
     }
 };
 
-template <int BT, bool X3, int BK>
+/* SWAP: the MFMA operands exchanged (acc holds C's blocks transposed) for
+ * row-view outputs, whose epilogue then reads and writes 16-byte runs */
+template <int BT, bool X3, int BK, bool SWAP>
 __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
 {
     constexpr int P = BK + 8;
@@ -393,11 +395,19 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
             for (int i = 0; i < FM; ++i)
 #pragma unroll
                 for (int j = 0; j < FM; ++j) {
-                    if (X3) {
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fb[j], acc[i][j], 0, 0, 0);
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fbl[j], acc[i][j], 0, 0, 0);
+                    if (SWAP) {
+                        if (X3) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[j], fal[i], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fbl[j], fa[i], acc[i][j], 0, 0, 0);
+                        }
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[j], fa[i], acc[i][j], 0, 0, 0);
+                    } else {
+                        if (X3) {
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fal[i], fb[j], acc[i][j], 0, 0, 0);
+                            acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fbl[j], acc[i][j], 0, 0, 0);
+                        }
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
                     }
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
                 }
         }
         if (more) {
@@ -414,9 +424,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
-        for (int j = 0; j < FM; ++j)
-            store_block32(acc[i][j], p.c, p.c_off, p.c_rs, p.c_cs, m0 + wm * WT + i * 32, n0 + wn * WT + j * 32,
-                          p.M, p.N, p.lower != 0, p.alpha, p.beta, lane);
+        for (int j = 0; j < FM; ++j) {
+            if (SWAP)
+                store_block32_t(acc[i][j], p.c, p.c_off, p.c_rs, p.c_cs, m0 + wm * WT + i * 32, n0 + wn * WT + j * 32,
+                                p.M, p.N, p.lower != 0, p.alpha, p.beta, lane);
+            else
+                store_block32(acc[i][j], p.c, p.c_off, p.c_rs, p.c_cs, m0 + wm * WT + i * 32, n0 + wn * WT + j * 32,
+                              p.M, p.N, p.lower != 0, p.alpha, p.beta, lane);
+        }
 }
 
 template <int BT, bool X3, int BK>
@@ -436,7 +451,10 @@ hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
     if (p.ntiles <= 0) return hipSuccess;
     if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    /* row view read and written (beta != 0): transposed accumulators, 16-byte runs along rows */
+    const bool swap = g.vc.cs == 1 && g.vc.rs != 1 && g.beta != 0.0;
+    if (swap) hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, true>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    else      hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, false>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

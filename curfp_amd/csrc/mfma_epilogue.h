@@ -47,3 +47,38 @@ __device__ __forceinline__ void store_block32(const ACC &acc, float *c, int64_t 
         }
     }
 }
+
+/* The same for an accumulator that holds the block transposed (the MFMA
+ * operands exchanged): element 4 g + e of lane l is row l % 32 and column
+ * 8 g + 4 (l / 32) + e, so a lane owns runs of 4 consecutive columns of one
+ * row — one 16-byte access when the view's columns are contiguous (a row
+ * view). */
+template <typename ACC>
+__device__ __forceinline__ void store_block32_t(const ACC &acc, float *c, int64_t c_off, int64_t c_rs, int64_t c_cs,
+                                                int64_t row0, int64_t col0, int64_t M, int64_t N, bool lower,
+                                                float alpha, float beta, int lane)
+{
+    const int64_t gi = row0 + (lane & 31);
+    const bool use_beta = beta != 0.f;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const int64_t gj0 = col0 + 8 * g + 4 * (lane >> 5);
+        if (c_cs == 1 && gi < M && gj0 + 3 < N && (!lower || gi >= gj0 + 3)) {
+            float *cp = c + c_off + gi * c_rs + gj0;
+            f32x4run v = {alpha * acc[4 * g], alpha * acc[4 * g + 1], alpha * acc[4 * g + 2], alpha * acc[4 * g + 3]};
+            if (use_beta) v = v + beta * *(const f32x4run *)cp;
+            *(f32x4run *)cp = v;
+            continue;
+        }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int64_t gj = gj0 + e;
+            if (gi < M && gj < N && (!lower || gi >= gj)) {
+                float *cp = c + c_off + gi * c_rs + gj * c_cs;
+                float val = alpha * acc[4 * g + e];
+                if (use_beta) val += beta * (*cp);
+                *cp = val;
+            }
+        }
+    }
+}
