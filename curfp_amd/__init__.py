@@ -63,6 +63,8 @@ try:
         dlansf as _dlansf_C,
         ssfmm as _ssfmm_C,
         dsfmm as _dsfmm_C,
+        spftri as _spftri_C,
+        dpftri as _dpftri_C,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(
@@ -515,6 +517,38 @@ def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R
 
 
 # ---------------------------------------------------------------------------
+# Inverse from the RFP Cholesky factor — curfp/__init__.py:392-404, :661-690
+# ---------------------------------------------------------------------------
+def spftri_raw(handle: Handle, transr: int, uplo: int, n: int, A: torch.Tensor) -> None:
+    """Low-level: overwrite A (the RFP Cholesky factor from spftrf) with inv(C)."""
+    _validate(A, "A")
+    _spftri_C(handle, transr, uplo, n, A.data_ptr())
+
+
+def spftri(C: torch.Tensor, *, n: int = None, transr: str = "T", uplo: str = "U") -> None:
+    """inv(C) in place from the RFP Cholesky factor produced by spftrf (same
+    transr / uplo), as L^-T L^-1 in the same RFP layout."""
+    _validate(C, "C")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    _spftri_C(_get_handle(), _op(transr), _fill(uplo), n, C.data_ptr())
+
+
+def dpftri_raw(handle: Handle, transr: int, uplo: int, n: int, A: torch.Tensor) -> None:
+    """FP64 twin of spftri_raw."""
+    _validate(A, "A", torch.float64)
+    _dpftri_C(handle, transr, uplo, n, A.data_ptr())
+
+
+def dpftri(C: torch.Tensor, *, n: int = None, transr: str = "T", uplo: str = "U") -> None:
+    """FP64 twin of spftri."""
+    _validate(C, "C", torch.float64)
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    _dpftri_C(_get_handle(), _op(transr), _fill(uplo), n, C.data_ptr())
+
+
+# ---------------------------------------------------------------------------
 # Symmetric RFP matrix times a dense block — curfp/__init__.py:1203-1360
 # ---------------------------------------------------------------------------
 _SIDE_MAP = {"L": SIDE_LEFT, "R": SIDE_RIGHT}
@@ -659,8 +693,8 @@ def _not_built(name):
     return fn
 
 
-for _name in ("spftri", "ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "spftrf_rcond",
-              "dpftri", "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dpftrf_rcond"):
+for _name in ("ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "spftrf_rcond",
+              "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dpftrf_rcond"):
     globals()[_name] = _not_built(_name)
     globals()[_name + "_raw"] = _not_built(_name + "_raw")
 del _name

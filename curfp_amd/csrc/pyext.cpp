@@ -195,6 +195,12 @@ PYBIND11_MODULE(_curfp_C, m)
     m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
         check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
     });
+    m.def("spftri", [](Handle &h, int transr, int uplo, int n, int64_t arf) {
+        check_status(curfpSpftri(h.get(), OPS(transr), FILL(uplo), n, P<float>(arf)));
+    });
+    m.def("dpftri", [](Handle &h, int transr, int uplo, int n, int64_t arf) {
+        check_status(curfpDpftri(h.get(), OPS(transr), FILL(uplo), n, P<double>(arf)));
+    });
     m.def("ssfmm", [](Handle &h, int transr, int uplo, int side, int m, int n, float alpha, int64_t arf, int64_t B,
                       int ldb, float beta, int64_t C, int ldc) {
         check_status(curfpSsfmm(h.get(), OPS(transr), FILL(uplo), static_cast<curfpSideMode_t>(side), m, n, &alpha,

@@ -772,6 +772,147 @@ static curfpStatus_t add_diag_impl(curfpHandle_t h, curfpOperation_t transr, cur
 }
 
 /* ------------------------------------------------------------------------ */
+/* spftri / dpftri                                                           */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * inv(C) = L^-T L^-1 from the RFP factor, in place (reference
+ * src/curfp_spftri.cu:159-388; LAPACK's ?pftri: the triangular inverse, then
+ * the product).  On the lower-factor views L = [L11 0; L21 L22]:
+ *   X11 = inv(L11);  L21 := -L21 X11;  X22 = inv(L22);  L21 := X22 L21;
+ *   C11 = X11^T X11 + X21^T X21;  C21 = X22^T X21;  C22 = X22^T X22.
+ * TRTRI, TRMM and LAUUM are halved recursively down to leaves; the
+ * off-diagonal pieces are view GEMMs, a leaf inverse comes from the batched
+ * leaf-inverse kernel, and the leaf TRMM / LAUUM are (in-place) GEMMs against
+ * a dense zero-padded copy of the leaf triangle.
+ */
+template <typename T>
+struct Inverter {
+    curfpHandle_t h;
+    T *a;          /* RFP array */
+    T *W, *V;      /* leaf x leaf scratch: dense triangle, leaf inverse */
+    int64_t leaf;
+
+    /* C (view c of the RFP array) = alpha X Y^T + beta C */
+    hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t N, int64_t K,
+                    double alpha, double beta, bool lower, int inplace)
+    {
+        if (K <= 0 && beta == 1.0) return hipSuccess;
+        GemmArgs g = gemm_args();
+        g.a = x; g.va = vx;
+        g.b = y; g.vb = vy;
+        g.c = a; g.vc = c;
+        g.M = M; g.N = N; g.K = K;
+        g.alpha = alpha; g.beta = beta;
+        g.lower = lower;
+        g.inplace = inplace;
+        return run_gemm<T>(h, g, Prec<T>::f64 ? RFP_MATH_F64 : RFP_MATH_F32);
+    }
+    /* W := lower triangle of the m x m view t (row-major, pitch leaf, zeros elsewhere) */
+    hipError_t dense(RfpView t, int64_t m)
+    {
+        if constexpr (Prec<T>::f64) return launch_tri_to_dense_f64(a, t, (int)m, W, (int)leaf, h->stream);
+        else                        return launch_tri_to_dense_f32(a, t, (int)m, W, (int)leaf, h->stream);
+    }
+    /* B (mb x m) := alpha B T, T lower m x m */
+    hipError_t trmm_right(RfpView B, int64_t mb, RfpView t, int64_t m, double alpha)
+    {
+        if (m <= 0 || mb <= 0) return hipSuccess;
+        if (m <= leaf) {
+            RFP_TRY(dense(t, m));
+            /* C = alpha B Y^T, Y(j, k) = T(k, j) = W[k*leaf + j]; C aliases B */
+            return gemm(B, a, B, W, RfpView{0, 1, leaf}, mb, m, m, alpha, 0.0, false, 1);
+        }
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        RFP_TRY(trmm_right(B, mb, t, m1, alpha));                                                   /* B1 := B1 T11 */
+        RFP_TRY(gemm(B, a, B.sub(0, m1), a, t.sub(m1, 0).t(), mb, m1, m2, alpha, 1.0, false, 0));  /* B1 += B2 T21 */
+        return trmm_right(B.sub(0, m1), mb, t.sub(m1, m1), m2, alpha);                              /* B2 := B2 T22 */
+    }
+    /* B (m x nb) := alpha op(T) B, T lower m x m, op(T) = T or T^T */
+    hipError_t trmm_left(RfpView B, int64_t nb, RfpView t, int64_t m, bool trans, double alpha)
+    {
+        if (m <= 0 || nb <= 0) return hipSuccess;
+        if (m <= leaf) {
+            RFP_TRY(dense(t, m));
+            /* C = alpha op(T) Y^T with Y = B^T; C aliases B */
+            const RfpView wx = trans ? RfpView{0, 1, leaf} : RfpView{0, leaf, 1};
+            return gemm(B, W, wx, a, B.t(), m, nb, m, alpha, 0.0, false, 2);
+        }
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        const RfpView B2 = B.sub(m1, 0);
+        if (!trans) {
+            RFP_TRY(trmm_left(B2, nb, t.sub(m1, m1), m2, false, alpha));                           /* B2 := T22 B2 */
+            RFP_TRY(gemm(B2, a, t.sub(m1, 0), a, B.t(), m2, nb, m1, alpha, 1.0, false, 0));        /* B2 += T21 B1 */
+            return trmm_left(B, nb, t, m1, false, alpha);                                          /* B1 := T11 B1 */
+        }
+        RFP_TRY(trmm_left(B, nb, t, m1, true, alpha));                                              /* B1 := T11^T B1 */
+        RFP_TRY(gemm(B, a, t.sub(m1, 0).t(), a, B2.t(), m1, nb, m2, alpha, 1.0, false, 0));        /* B1 += T21^T B2 */
+        return trmm_left(B2, nb, t.sub(m1, m1), m2, true, alpha);                                   /* B2 := T22^T B2 */
+    }
+    /* T := inv(T) in place, T lower m x m */
+    hipError_t trtri(RfpView t, int64_t m)
+    {
+        if (m <= 0) return hipSuccess;
+        if (m <= leaf) {
+            RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)m * m * m / 3.0, 0.0,
+                          [&] { return Prec<T>::trtri(a, t, m, V, h->stream); }));
+            if constexpr (Prec<T>::f64) return launch_dense_to_tri_f64(V, (int)leaf, a, t, (int)m, h->stream);
+            else                        return launch_dense_to_tri_f32(V, (int)leaf, a, t, (int)m, h->stream);
+        }
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        RFP_TRY(trtri(t, m1));
+        RFP_TRY(trmm_right(t.sub(m1, 0), m2, t, m1, -1.0));                                        /* T21 := -T21 X11 */
+        RFP_TRY(trtri(t.sub(m1, m1), m2));
+        return trmm_left(t.sub(m1, 0), m1, t.sub(m1, m1), m2, false, 1.0);                          /* T21 := X22 T21 */
+    }
+    /* T := lower(T^T T) in place, T lower m x m */
+    hipError_t lauum(RfpView t, int64_t m)
+    {
+        if (m <= 0) return hipSuccess;
+        if (m <= leaf) {
+            RFP_TRY(dense(t, m));
+            /* C = X Y^T (lower) with X = Y = T^T, X(i, k) = T(k, i) = W[k*leaf + i] */
+            return gemm(t, W, RfpView{0, 1, leaf}, W, RfpView{0, 1, leaf}, m, m, m, 1.0, 0.0, true, 0);
+        }
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        const RfpView t21 = t.sub(m1, 0);
+        RFP_TRY(lauum(t, m1));                                                                      /* T11 := T11^T T11 */
+        RFP_TRY(gemm(t, a, t21.t(), a, t21.t(), m1, m1, m2, 1.0, 1.0, true, 0));                   /* T11 += T21^T T21 */
+        RFP_TRY(trmm_left(t21, m1, t.sub(m1, m1), m2, true, 1.0));                                 /* T21 := T22^T T21 */
+        return lauum(t.sub(m1, m1), m2);                                                           /* T22 := T22^T T22 */
+    }
+};
+
+template <typename T>
+static curfpStatus_t pftri_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, T *A)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!A) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const int64_t leaf = rfp_leaf(Prec<T>::f64);
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, (size_t)2 * leaf * leaf * sizeof(T), &ws);
+    if (e != hipSuccess) return st_of(e);
+    Inverter<T> iv{h, A, (T *)ws, (T *)ws + leaf * leaf, leaf};
+    const int64_t n1 = g.n1, n2 = g.n2;
+    /* L^-1 over the views (the first split of the recursion at n1) */
+    if ((e = iv.trtri(g.v11, n1)) != hipSuccess) return st_of(e);
+    if ((e = iv.trmm_right(g.v21, n2, g.v11, n1, -1.0)) != hipSuccess) return st_of(e);
+    if ((e = iv.trtri(g.v22, n2)) != hipSuccess) return st_of(e);
+    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, false, 1.0)) != hipSuccess) return st_of(e);
+    /* L^-T L^-1 */
+    if ((e = iv.lauum(g.v11, n1)) != hipSuccess) return st_of(e);
+    if ((e = iv.gemm(g.v11, A, g.v21.t(), A, g.v21.t(), n1, n1, n2, 1.0, 1.0, true, 0)) != hipSuccess) return st_of(e);
+    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, true, 1.0)) != hipSuccess) return st_of(e);
+    return st_of(iv.lauum(g.v22, n2));
+}
+
+/* ------------------------------------------------------------------------ */
 /* slansf / dlansf                                                           */
 /* ------------------------------------------------------------------------ */
 
@@ -1154,6 +1295,12 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curf
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
                                   const double *value, double *arf)
 { return add_diag_impl<double>(h, transr, uplo, n, value, arf); }
+
+curfpStatus_t curfpSpftri(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, float *arf)
+{ return pftri_impl<float>(h, transr, uplo, n, arf); }
+
+curfpStatus_t curfpDpftri(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, double *arf)
+{ return pftri_impl<double>(h, transr, uplo, n, arf); }
 
 curfpStatus_t curfpSsfmm(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpSideMode_t side,
                          int m, int n, const float *alpha, const float *arf, const float *B, int ldb,

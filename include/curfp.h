@@ -151,6 +151,13 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const float *value, float *arf);
 
+/* curfpSpftri — reference include/curfp.h:175-186, src/curfp_spftri.cu:159-388.
+ * Overwrites the RFP Cholesky factor from curfpSpftrf with inv(C) in the same
+ * RFP layout (L^-1 in place, then L^-T L^-1).  Asynchronous on the handle
+ * stream. */
+curfpStatus_t curfpSpftri(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, float *arf);
+
 /* curfpSsfmm — reference include/curfp.h:322-348, src/curfp_ssfmm.cpp:107-306.
  * side L: C := alpha*A*B + beta*C (A m x m); side R: C := alpha*B*A + beta*C
  * (A n x n); A symmetric in RFP, B and C column-major m x n (ldb, ldc >= m),
@@ -300,6 +307,10 @@ curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const double *value, double *arf);
+
+/* curfpDpftri — reference include/curfp.h:427-436.  FP64 twin of curfpSpftri. */
+curfpStatus_t curfpDpftri(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, double *arf);
 
 /* curfpDsfmm — reference include/curfp.h:528-544, src/curfp_dsfmm.cpp:78-219.  FP64 twin of
  * curfpSsfmm. */

@@ -38,6 +38,9 @@ Functions and what pins each one (see tests/test_oracle.py):
         the dense product of an independently built symmetric matrix, the
         known answer of S = I, alpha = 0 / beta = 0 cases, and a planted
         one-triangle mistake.
+  pftri  — reference src/curfp_spftri.cu:159-388 (inverse of the triangular
+        factor, then L^-T L^-1); pinned against LAPACK ?pftri (scipy), an
+        independent RFP routine, and C inv(C) = I.
   lansf  — reference src/curfp_slansf.cu:273-435 (max |a|, largest absolute
         column sum, Frobenius norm of the symmetric matrix); pinned by
         LAPACK ?lange on the dense matrix (an independent routine), the
@@ -52,7 +55,7 @@ from scipy.linalg import lapack as _lapack
 
 __all__ = [
     "views", "lower_index_map", "trttf", "tfttr", "diag_indices", "add_to_diagonal",
-    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "lansf", "sfmm", "chain_flops",
+    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "pftri", "lansf", "sfmm", "chain_flops",
 ]
 
 
@@ -326,6 +329,20 @@ def pftrs(n, arf, B, transr="T", uplo="U"):
         b1 = trsm(1.0, L11, b1, side=0, lower=1, trans_a=1)
     X = np.concatenate([b1, b2], axis=0).astype(dt)
     return X[:, 0] if squeeze else X
+
+
+def pftri(n: int, arf: np.ndarray, transr: str = "T", uplo: str = "U") -> np.ndarray:
+    """inv(C) in RFP from the RFP Cholesky factor (reference
+    src/curfp_spftri.cu:159-388): X = inv(L) (LAPACK ?trtri on the dense
+    factor), inv(C) = X^T X, packed back into the same layout; float64."""
+    if n == 0:
+        return np.zeros(0)
+    L = rfp_lower_factor(np.asarray(arf, dtype=np.float64), n, transr, uplo)
+    X, info = _lapack.dtrtri(L, lower=1)
+    if info != 0:
+        raise np.linalg.LinAlgError(f"singular factor (trtri info {info})")
+    X = np.tril(X)
+    return dense_sym_to_rfp(X.T @ X, transr, uplo)
 
 
 def lansf(arf: np.ndarray, n: int, norm: str, transr: str = "T", uplo: str = "U") -> float:

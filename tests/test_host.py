@@ -106,4 +106,4 @@ def test_python_geometry_matches_oracle():
     with pytest.raises(ValueError):
         curfp.ssfrk(torch.zeros(4, 2), torch.zeros(10))  # not a device tensor
     with pytest.raises(NotImplementedError):
-        curfp.spftri(None)
+        curfp.spfcon(None, 1.0)

@@ -298,3 +298,38 @@ def test_planted_sfmm_mistake_is_caught(monkeypatch):
     real = O.rfp_to_dense_sym
     monkeypatch.setattr(O, "rfp_to_dense_sym", lambda a, m, transr="T", uplo="U": np.tril(real(a, m, transr, uplo)))
     assert not np.allclose(O.sfmm(arf, n, B, None, 1.0, 0.0, "T", "U", "L"), S @ B)
+
+
+# --------------------------------------------------------------------------- inverse
+
+@pytest.mark.parametrize("n", [1, 2, 3, 6, 17, 64, 129])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_pftri_matches_lapack_pftri(n, t, u):
+    from scipy.linalg import lapack
+
+    rng = np.random.default_rng(n)
+    X = rng.standard_normal((n, n))
+    S = X @ X.T + n * np.eye(n)
+    F, info = lapack.dpftrf(n, O.trttf(S, t, u), transr=t, uplo=u)
+    assert info == 0
+    ref, info = lapack.dpftri(n, F.copy(), transr=t, uplo=u)
+    assert info == 0
+    got = O.pftri(n, F, t, u)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    assert np.linalg.norm(S @ O.rfp_to_dense_sym(got, n, t, u) - np.eye(n)) < 1e-12
+
+
+def test_planted_pftri_mistake_is_caught(monkeypatch):
+    """Inverting the transposed factor (the wrong triangle) must fail the
+    LAPACK comparison."""
+    from scipy.linalg import lapack
+
+    n = 7
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((n, n))
+    S = X @ X.T + n * np.eye(n)
+    F, _ = lapack.dpftrf(n, O.trttf(S, "N", "L"), transr="N", uplo="L")
+    ref, _ = lapack.dpftri(n, F.copy(), transr="N", uplo="L")
+    real = O.rfp_lower_factor
+    monkeypatch.setattr(O, "rfp_lower_factor", lambda a, m, transr="T", uplo="U": real(a, m, transr, uplo).T)
+    assert np.abs(O.pftri(n, F, "N", "L") - ref).max() > 1e-6
