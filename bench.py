@@ -238,6 +238,44 @@ def run_norms(curfp, torch, args, dev) -> int:
     return 0
 
 
+def run_pftri(curfp, torch, args, dev) -> int:
+    """spftri on the FP32 factor of C = A A^T + lam I (the chain's matrix):
+    2 n^3 / 3 flops per call (the triangular inverse and the L^-T L^-1
+    product), FP32 view GEMMs on the matrix units."""
+    n, k, lam = args.n, args.k, args.lam
+    A, _ = make_inputs(n, k, 1, args.seed, dev)
+    C = torch.empty(n * (n + 1) // 2, dtype=torch.float32, device=dev)
+    curfp.ssfrk(A, C)
+    curfp.add_to_diagonal(C, lam)
+    if curfp.spftrf(C, check=False) != 0:
+        raise RuntimeError("factorisation failed")
+    F = C.clone()
+    steps = max(args.steps, 1)
+    for _ in range(max(args.warmup, 0)):
+        C.copy_(F)
+        curfp.spftri(C)
+    times = []
+    for _ in range(steps):
+        C.copy_(F)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        curfp.spftri(C)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    ms = sorted(times)[len(times) // 2] * 1e3
+    fl = 2.0 * n ** 3 / 3.0
+    tf = fl / (ms * 1e-3) / 1e12
+    log(f"spftri n={n}: {ms:.1f} ms, {tf:.1f} TFLOP/s")
+    line = {"metric": "spftri_tflops", "value": tf, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic", "config": {"workload": f"spftri on the FP32 factor of A A^T + lam I, n={n}",
+                                            "n": n, "k": k, "lambda": lam},
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+                         "traffic": None, "note": "median wall time per call, 2 n^3 / 3 flops"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_sfmm(curfp, torch, args, dev) -> int:
     """ssfmm C := A B (side L) with A a random n-order RFP matrix and B a
     column-major n x nrhs block: 2 n^2 nrhs flops per call on the FP32 matrix
@@ -290,9 +328,9 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm"], default="both",
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri"], default="both",
                     help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
-                         "sfmm: time ssfmm (side L, nrhs columns)")
+                         "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
@@ -317,6 +355,8 @@ def main() -> int:
         return run_norms(curfp, torch, args, dev)
     if args.mode == "sfmm":
         return run_sfmm(curfp, torch, args, dev)
+    if args.mode == "pftri":
+        return run_pftri(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
     fl = chain_flops(n, k, nrhs)
     log(f"inputs n={n} k={k} nrhs={nrhs} seed={args.seed}")
