@@ -156,56 +156,8 @@ __global__ void fill_zero_kernel(uint32_t *p, int64_t words)
 
 __global__ void set_int_kernel(int *p, int v) { *p = v; }
 
-/* dst (ld x ld, row-major) := the lower triangle of the m x m view, zero
- * above the diagonal and outside m x m */
-template <typename T>
-__global__ void tri_to_dense_kernel(const T *a, RfpView v, int m, T *dst, int ld)
-{
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= ld * ld) return;
-    const int i = idx / ld, j = idx % ld;
-    dst[idx] = (i < m && j <= i) ? a[v.at(i, j)] : T(0);
-}
-
-/* lower triangle of the m x m view := the lower triangle of src (row-major, ld) */
-template <typename T>
-__global__ void dense_to_tri_kernel(const T *src, int ld, T *a, RfpView v, int m)
-{
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= m * m) return;
-    const int i = idx / m, j = idx % m;
-    if (j <= i) a[v.at(i, j)] = src[i * ld + j];
-}
-
-template <typename T>
-hipError_t tri_to_dense(const T *a, RfpView v, int m, T *dst, int ld, hipStream_t s)
-{
-    if (m > ld) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((tri_to_dense_kernel<T>), dim3((unsigned)((ld * ld + 255) / 256)), dim3(256), 0, s, a, v, m,
-                       dst, ld);
-    return hipGetLastError();
-}
-
-template <typename T>
-hipError_t dense_to_tri(const T *src, int ld, T *a, RfpView v, int m, hipStream_t s)
-{
-    if (m > ld) return hipErrorInvalidValue;
-    if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL((dense_to_tri_kernel<T>), dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, s, src, ld, a,
-                       v, m);
-    return hipGetLastError();
-}
-
 }  // namespace
 
-hipError_t launch_tri_to_dense_f32(const float *a, RfpView v, int m, float *dst, int ld, hipStream_t s)
-{ return tri_to_dense<float>(a, v, m, dst, ld, s); }
-hipError_t launch_tri_to_dense_f64(const double *a, RfpView v, int m, double *dst, int ld, hipStream_t s)
-{ return tri_to_dense<double>(a, v, m, dst, ld, s); }
-hipError_t launch_dense_to_tri_f32(const float *src, int ld, float *a, RfpView v, int m, hipStream_t s)
-{ return dense_to_tri<float>(src, ld, a, v, m, s); }
-hipError_t launch_dense_to_tri_f64(const double *src, int ld, double *a, RfpView v, int m, hipStream_t s)
-{ return dense_to_tri<double>(src, ld, a, v, m, s); }
 
 hipError_t launch_trttf_f32(const RfpGeom &g, bool lower, const float *A, int64_t lda, float *arf, hipStream_t s)
 { return rfp_copy<float>(A, arf, make_copy(g, lower, lda, true), s); }

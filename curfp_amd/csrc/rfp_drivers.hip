@@ -782,17 +782,20 @@ static curfpStatus_t add_diag_impl(curfpHandle_t h, curfpOperation_t transr, cur
  *   X11 = inv(L11);  L21 := -L21 X11;  X22 = inv(L22);  L21 := X22 L21;
  *   C11 = X11^T X11 + X21^T X21;  C21 = X22^T X21;  C22 = X22^T X22.
  * TRTRI, TRMM and LAUUM are halved recursively down to leaves; the
- * off-diagonal pieces are view GEMMs, a leaf inverse comes from the batched
- * leaf-inverse kernel, and the leaf TRMM / LAUUM are (in-place) GEMMs against
- * a dense zero-padded copy of the leaf triangle.
+ * off-diagonal pieces are view GEMMs.  Every leaf operand of the recursion is
+ * an inverted diagonal tile not yet overwritten by LAUUM, so the inverses of
+ * all diagonal tiles are computed up front by the batched leaf-inverse kernel
+ * (dense, row-major, zero above the diagonal) and serve directly as the
+ * leaf TRMM / LAUUM operands; `slot` indexes them (v11's leaves, then v22's).
  */
 template <typename T>
 struct Inverter {
     curfpHandle_t h;
     T *a;          /* RFP array */
-    T *W, *V;      /* leaf x leaf scratch: dense triangle, leaf inverse */
+    T *inv;        /* leaf x leaf inverse of diagonal tile `slot` at inv + slot*leaf*leaf */
     int64_t leaf;
 
+    const T *tile(int64_t slot) const { return inv + slot * leaf * leaf; }
     /* C (view c of the RFP array) = alpha X Y^T + beta C */
     hipError_t gemm(RfpView c, const T *x, RfpView vx, const T *y, RfpView vy, int64_t M, int64_t N, int64_t K,
                     double alpha, double beta, bool lower, int inplace)
@@ -808,78 +811,60 @@ struct Inverter {
         g.inplace = inplace;
         return run_gemm<T>(h, g, Prec<T>::f64 ? RFP_MATH_F64 : RFP_MATH_F32);
     }
-    /* W := lower triangle of the m x m view t (row-major, pitch leaf, zeros elsewhere) */
-    hipError_t dense(RfpView t, int64_t m)
-    {
-        if constexpr (Prec<T>::f64) return launch_tri_to_dense_f64(a, t, (int)m, W, (int)leaf, h->stream);
-        else                        return launch_tri_to_dense_f32(a, t, (int)m, W, (int)leaf, h->stream);
-    }
-    /* B (mb x m) := alpha B T, T lower m x m */
-    hipError_t trmm_right(RfpView B, int64_t mb, RfpView t, int64_t m, double alpha)
+    /* B (mb x m) := alpha B X, X = inverted lower m x m (diagonal leaves at slot ..) */
+    hipError_t trmm_right(RfpView B, int64_t mb, RfpView t, int64_t m, double alpha, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return hipSuccess;
-        if (m <= leaf) {
-            RFP_TRY(dense(t, m));
-            /* C = alpha B Y^T, Y(j, k) = T(k, j) = W[k*leaf + j]; C aliases B */
-            return gemm(B, a, B, W, RfpView{0, 1, leaf}, mb, m, m, alpha, 0.0, false, 1);
-        }
+        if (m <= leaf)   /* C = alpha B Y^T, Y(j, k) = X(k, j); C aliases B */
+            return gemm(B, a, B, tile(slot), RfpView{0, 1, leaf}, mb, m, m, alpha, 0.0, false, 1);
         const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
-        RFP_TRY(trmm_right(B, mb, t, m1, alpha));                                                   /* B1 := B1 T11 */
-        RFP_TRY(gemm(B, a, B.sub(0, m1), a, t.sub(m1, 0).t(), mb, m1, m2, alpha, 1.0, false, 0));  /* B1 += B2 T21 */
-        return trmm_right(B.sub(0, m1), mb, t.sub(m1, m1), m2, alpha);                              /* B2 := B2 T22 */
+        RFP_TRY(trmm_right(B, mb, t, m1, alpha, slot));                                            /* B1 := B1 X11 */
+        RFP_TRY(gemm(B, a, B.sub(0, m1), a, t.sub(m1, 0).t(), mb, m1, m2, alpha, 1.0, false, 0));  /* B1 += B2 X21 */
+        return trmm_right(B.sub(0, m1), mb, t.sub(m1, m1), m2, alpha, slot + m1 / leaf);           /* B2 := B2 X22 */
     }
-    /* B (m x nb) := alpha op(T) B, T lower m x m, op(T) = T or T^T */
-    hipError_t trmm_left(RfpView B, int64_t nb, RfpView t, int64_t m, bool trans, double alpha)
+    /* B (m x nb) := alpha op(X) B, X inverted lower m x m, op(X) = X or X^T */
+    hipError_t trmm_left(RfpView B, int64_t nb, RfpView t, int64_t m, bool trans, double alpha, int64_t slot)
     {
         if (m <= 0 || nb <= 0) return hipSuccess;
-        if (m <= leaf) {
-            RFP_TRY(dense(t, m));
-            /* C = alpha op(T) Y^T with Y = B^T; C aliases B */
+        if (m <= leaf) { /* C = alpha op(X) Y^T with Y = B^T; C aliases B */
             const RfpView wx = trans ? RfpView{0, 1, leaf} : RfpView{0, leaf, 1};
-            return gemm(B, W, wx, a, B.t(), m, nb, m, alpha, 0.0, false, 2);
+            return gemm(B, tile(slot), wx, a, B.t(), m, nb, m, alpha, 0.0, false, 2);
         }
-        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1, s2 = slot + m1 / leaf;
         const RfpView B2 = B.sub(m1, 0);
         if (!trans) {
-            RFP_TRY(trmm_left(B2, nb, t.sub(m1, m1), m2, false, alpha));                           /* B2 := T22 B2 */
-            RFP_TRY(gemm(B2, a, t.sub(m1, 0), a, B.t(), m2, nb, m1, alpha, 1.0, false, 0));        /* B2 += T21 B1 */
-            return trmm_left(B, nb, t, m1, false, alpha);                                          /* B1 := T11 B1 */
+            RFP_TRY(trmm_left(B2, nb, t.sub(m1, m1), m2, false, alpha, s2));                       /* B2 := X22 B2 */
+            RFP_TRY(gemm(B2, a, t.sub(m1, 0), a, B.t(), m2, nb, m1, alpha, 1.0, false, 0));        /* B2 += X21 B1 */
+            return trmm_left(B, nb, t, m1, false, alpha, slot);                                    /* B1 := X11 B1 */
         }
-        RFP_TRY(trmm_left(B, nb, t, m1, true, alpha));                                              /* B1 := T11^T B1 */
-        RFP_TRY(gemm(B, a, t.sub(m1, 0).t(), a, B2.t(), m1, nb, m2, alpha, 1.0, false, 0));        /* B1 += T21^T B2 */
-        return trmm_left(B2, nb, t.sub(m1, m1), m2, true, alpha);                                   /* B2 := T22^T B2 */
+        RFP_TRY(trmm_left(B, nb, t, m1, true, alpha, slot));                                        /* B1 := X11^T B1 */
+        RFP_TRY(gemm(B, a, t.sub(m1, 0).t(), a, B2.t(), m1, nb, m2, alpha, 1.0, false, 0));        /* B1 += X21^T B2 */
+        return trmm_left(B2, nb, t.sub(m1, m1), m2, true, alpha, s2);                               /* B2 := X22^T B2 */
     }
-    /* T := inv(T) in place, T lower m x m */
-    hipError_t trtri(RfpView t, int64_t m)
+    /* off-diagonal blocks of T := inv(T), T lower m x m (the diagonal leaves
+     * are already inverted in `inv`) */
+    hipError_t trtri(RfpView t, int64_t m, int64_t slot)
+    {
+        if (m <= leaf) return hipSuccess;
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1, s2 = slot + m1 / leaf;
+        RFP_TRY(trtri(t, m1, slot));
+        RFP_TRY(trmm_right(t.sub(m1, 0), m2, t, m1, -1.0, slot));                                  /* T21 := -T21 X11 */
+        RFP_TRY(trtri(t.sub(m1, m1), m2, s2));
+        return trmm_left(t.sub(m1, 0), m1, t.sub(m1, m1), m2, false, 1.0, s2);                      /* T21 := X22 T21 */
+    }
+    /* T := lower(X^T X) in place, X the inverted lower m x m */
+    hipError_t lauum(RfpView t, int64_t m, int64_t slot)
     {
         if (m <= 0) return hipSuccess;
-        if (m <= leaf) {
-            RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)m * m * m / 3.0, 0.0,
-                          [&] { return Prec<T>::trtri(a, t, m, V, h->stream); }));
-            if constexpr (Prec<T>::f64) return launch_dense_to_tri_f64(V, (int)leaf, a, t, (int)m, h->stream);
-            else                        return launch_dense_to_tri_f32(V, (int)leaf, a, t, (int)m, h->stream);
-        }
-        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
-        RFP_TRY(trtri(t, m1));
-        RFP_TRY(trmm_right(t.sub(m1, 0), m2, t, m1, -1.0));                                        /* T21 := -T21 X11 */
-        RFP_TRY(trtri(t.sub(m1, m1), m2));
-        return trmm_left(t.sub(m1, 0), m1, t.sub(m1, m1), m2, false, 1.0);                          /* T21 := X22 T21 */
-    }
-    /* T := lower(T^T T) in place, T lower m x m */
-    hipError_t lauum(RfpView t, int64_t m)
-    {
-        if (m <= 0) return hipSuccess;
-        if (m <= leaf) {
-            RFP_TRY(dense(t, m));
-            /* C = X Y^T (lower) with X = Y = T^T, X(i, k) = T(k, i) = W[k*leaf + i] */
-            return gemm(t, W, RfpView{0, 1, leaf}, W, RfpView{0, 1, leaf}, m, m, m, 1.0, 0.0, true, 0);
-        }
-        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1;
+        if (m <= leaf)   /* C = Y Y^T (lower) with Y = X^T, Y(i, k) = X(k, i) */
+            return gemm(t, tile(slot), RfpView{0, 1, leaf}, tile(slot), RfpView{0, 1, leaf}, m, m, m, 1.0, 0.0,
+                        true, 0);
+        const int64_t m1 = leaf_split(m, leaf), m2 = m - m1, s2 = slot + m1 / leaf;
         const RfpView t21 = t.sub(m1, 0);
-        RFP_TRY(lauum(t, m1));                                                                      /* T11 := T11^T T11 */
-        RFP_TRY(gemm(t, a, t21.t(), a, t21.t(), m1, m1, m2, 1.0, 1.0, true, 0));                   /* T11 += T21^T T21 */
-        RFP_TRY(trmm_left(t21, m1, t.sub(m1, m1), m2, true, 1.0));                                 /* T21 := T22^T T21 */
-        return lauum(t.sub(m1, m1), m2);                                                           /* T22 := T22^T T22 */
+        RFP_TRY(lauum(t, m1, slot));                                                                /* T11 := X11^T X11 */
+        RFP_TRY(gemm(t, a, t21.t(), a, t21.t(), m1, m1, m2, 1.0, 1.0, true, 0));                   /* T11 += X21^T X21 */
+        RFP_TRY(trmm_left(t21, m1, t.sub(m1, m1), m2, true, 1.0, s2));                             /* T21 := X22^T X21 */
+        return lauum(t.sub(m1, m1), m2, s2);                                                       /* T22 := X22^T X22 */
     }
 };
 
@@ -895,21 +880,30 @@ static curfpStatus_t pftri_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     if (!A) return CURFP_STATUS_INVALID_VALUE;
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
     const int64_t leaf = rfp_leaf(Prec<T>::f64);
+    const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
     void *ws = nullptr;
-    hipError_t e = rfp_workspace(h, (size_t)2 * leaf * leaf * sizeof(T), &ws);
+    hipError_t e = rfp_workspace(h, (size_t)(s1 + s2) * leaf * leaf * sizeof(T), &ws);
     if (e != hipSuccess) return st_of(e);
-    Inverter<T> iv{h, A, (T *)ws, (T *)ws + leaf * leaf, leaf};
+    T *inv = (T *)ws;
+    /* inverses of every diagonal leaf, two launches */
+    e = timed(h, CURFP_KERNEL_TRTRI, (double)n * leaf * leaf / 3.0, 0.0, [&] {
+        hipError_t r = Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream);
+        if (r == hipSuccess) r = Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream);
+        return r;
+    });
+    if (e != hipSuccess) return st_of(e);
+    Inverter<T> iv{h, A, inv, leaf};
     const int64_t n1 = g.n1, n2 = g.n2;
     /* L^-1 over the views (the first split of the recursion at n1) */
-    if ((e = iv.trtri(g.v11, n1)) != hipSuccess) return st_of(e);
-    if ((e = iv.trmm_right(g.v21, n2, g.v11, n1, -1.0)) != hipSuccess) return st_of(e);
-    if ((e = iv.trtri(g.v22, n2)) != hipSuccess) return st_of(e);
-    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, false, 1.0)) != hipSuccess) return st_of(e);
+    if ((e = iv.trtri(g.v11, n1, 0)) != hipSuccess) return st_of(e);
+    if ((e = iv.trmm_right(g.v21, n2, g.v11, n1, -1.0, 0)) != hipSuccess) return st_of(e);
+    if ((e = iv.trtri(g.v22, n2, s1)) != hipSuccess) return st_of(e);
+    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, false, 1.0, s1)) != hipSuccess) return st_of(e);
     /* L^-T L^-1 */
-    if ((e = iv.lauum(g.v11, n1)) != hipSuccess) return st_of(e);
+    if ((e = iv.lauum(g.v11, n1, 0)) != hipSuccess) return st_of(e);
     if ((e = iv.gemm(g.v11, A, g.v21.t(), A, g.v21.t(), n1, n1, n2, 1.0, 1.0, true, 0)) != hipSuccess) return st_of(e);
-    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, true, 1.0)) != hipSuccess) return st_of(e);
-    return st_of(iv.lauum(g.v22, n2));
+    if ((e = iv.trmm_left(g.v21, n1, g.v22, n2, true, 1.0, s1)) != hipSuccess) return st_of(e);
+    return st_of(iv.lauum(g.v22, n2, s1));
 }
 
 /* ------------------------------------------------------------------------ */

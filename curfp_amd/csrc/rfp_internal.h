@@ -273,13 +273,6 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
                                   RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
                                   unsigned long long epoch, hipStream_t s);
-/* Leaf copies of spftri (pack.hip): dst (ld x ld row-major) := lower triangle
- * of the m x m view (zeros elsewhere); and back (lower triangle only). */
-hipError_t launch_tri_to_dense_f32(const float *a, RfpView v, int m, float *dst, int ld, hipStream_t s);
-hipError_t launch_tri_to_dense_f64(const double *a, RfpView v, int m, double *dst, int ld, hipStream_t s);
-hipError_t launch_dense_to_tri_f32(const float *src, int ld, float *a, RfpView v, int m, hipStream_t s);
-hipError_t launch_dense_to_tri_f64(const double *src, int ld, double *a, RfpView v, int m, hipStream_t s);
-
 /* C := alpha * A * Y + beta * C (trsv_block.hip) with A the n x n symmetric
  * matrix stored in RFP at A (geometry g), Y an n x N view and C an n x N view
  * (element (i, c) at off + i*rs + c*cs).  Y and C must not overlap.  FP32
