@@ -65,6 +65,12 @@ try:
         dsfmm as _dsfmm_C,
         spftri as _spftri_C,
         dpftri as _dpftri_C,
+        ssfr as _ssfr_C,
+        dsfr as _dsfr_C,
+        ssfr2 as _ssfr2_C,
+        dsfr2 as _dsfr2_C,
+        ssfr2k as _ssfr2k_C,
+        dsfr2k as _dsfr2k_C,
     )
 except ImportError as exc:  # pragma: no cover - exercised only without a build
     raise ImportError(
@@ -517,6 +523,154 @@ def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R
 
 
 # ---------------------------------------------------------------------------
+# Rank-1 / rank-2 / rank-2k updates — curfp/__init__.py:966-1200
+# ---------------------------------------------------------------------------
+def _vec(t: torch.Tensor, name: str, dtype) -> None:
+    if not (t.is_cuda and t.dtype == dtype and t.ndim == 1 and t.is_contiguous()):
+        raise ValueError(f"{name} must be a contiguous 1-D {dtype} CUDA tensor")
+
+
+def ssfr_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, x: torch.Tensor, incx: int,
+             arf: torch.Tensor) -> None:
+    """Low-level ssfr: arf := alpha x x^T + arf (x with BLAS increment incx)."""
+    _validate_strided(x, "x", torch.float32)
+    _validate(arf, "arf")
+    _ssfr_C(handle, transr, uplo, n, float(alpha), x.data_ptr(), incx, arf.data_ptr())
+
+
+def ssfr(arf: torch.Tensor, x: torch.Tensor, *, alpha: float = 1.0, transr: str = "T", uplo: str = "U",
+         n: int = None) -> None:
+    """Symmetric rank-1 update in RFP: arf := alpha x x^T + arf."""
+    _validate(arf, "arf")
+    _vec(x, "x", torch.float32)
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    _ssfr_C(_get_handle(), _op(transr), _fill(uplo), n, float(alpha), x.data_ptr(), 1, arf.data_ptr())
+
+
+def dsfr(arf: torch.Tensor, x: torch.Tensor, *, alpha: float = 1.0, transr: str = "T", uplo: str = "U",
+         n: int = None) -> None:
+    """FP64 twin of ssfr."""
+    _validate(arf, "arf", torch.float64)
+    _vec(x, "x", torch.float64)
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    _dsfr_C(_get_handle(), _op(transr), _fill(uplo), n, float(alpha), x.data_ptr(), 1, arf.data_ptr())
+
+
+def ssfr2_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, x: torch.Tensor, incx: int,
+              y: torch.Tensor, incy: int, arf: torch.Tensor) -> None:
+    """Low-level ssfr2: arf := alpha (x y^T + y x^T) + arf."""
+    _validate_strided(x, "x", torch.float32)
+    _validate_strided(y, "y", torch.float32)
+    _validate(arf, "arf")
+    _ssfr2_C(handle, transr, uplo, n, float(alpha), x.data_ptr(), incx, y.data_ptr(), incy, arf.data_ptr())
+
+
+def ssfr2(arf: torch.Tensor, x: torch.Tensor, y: torch.Tensor, *, alpha: float = 1.0, transr: str = "T",
+          uplo: str = "U", n: int = None) -> None:
+    """Symmetric rank-2 update in RFP: arf := alpha (x y^T + y x^T) + arf."""
+    _validate(arf, "arf")
+    _vec(x, "x", torch.float32)
+    _vec(y, "y", torch.float32)
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    _ssfr2_C(_get_handle(), _op(transr), _fill(uplo), n, float(alpha), x.data_ptr(), 1, y.data_ptr(), 1,
+             arf.data_ptr())
+
+
+def dsfr2(arf: torch.Tensor, x: torch.Tensor, y: torch.Tensor, *, alpha: float = 1.0, transr: str = "T",
+          uplo: str = "U", n: int = None) -> None:
+    """FP64 twin of ssfr2."""
+    _validate(arf, "arf", torch.float64)
+    _vec(x, "x", torch.float64)
+    _vec(y, "y", torch.float64)
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    _dsfr2_C(_get_handle(), _op(transr), _fill(uplo), n, float(alpha), x.data_ptr(), 1, y.data_ptr(), 1,
+             arf.data_ptr())
+
+
+def _sfr2k(fn, dtype, A, B, C, alpha, beta, transr, uplo, trans, n, k, lda, ldb):
+    _validate(A, "A", dtype)
+    _validate(B, "B", dtype)
+    _validate(C, "C", dtype)
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    t = trans.upper()
+    if A.ndim != 2 or B.shape != A.shape:
+        raise ValueError("A and B must be 2-D tensors of the same shape")
+    # trans='T': row-major (n, k) = column-major k x n (lda = k); trans='N': row-major (k, n) = column-major
+    # n x k (lda = n; the reference infers lda = A.shape[0] here, which its C entry rejects unless k >= n)
+    if k is None:
+        k = A.shape[1] if t == "T" else A.shape[0]
+    if lda is None:
+        lda = A.shape[1]
+    if ldb is None:
+        ldb = B.shape[1]
+    if n > 0 and k > 0:
+        for X, ld, name in ((A, lda, "A"), (B, ldb, "B")):
+            need = (n - 1) * ld + k if t == "T" else (k - 1) * ld + n
+            if X.numel() < need:
+                raise ValueError(f"{name} holds {X.numel()} elements; n={n}, k={k}, ld={ld} needs {need}")
+    fn(_get_handle(), _op(transr), _fill(uplo), _op(trans), n, k, float(alpha), A.data_ptr(), lda, B.data_ptr(),
+       ldb, float(beta), C.data_ptr())
+
+
+def ssfr2k(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, *, alpha: float = 1.0, beta: float = 0.0,
+           transr: str = "T", uplo: str = "U", trans: str = "T", n: int = None, k: int = None, lda: int = None,
+           ldb: int = None) -> None:
+    """Symmetric rank-2k update in RFP: C := alpha op(A) op(B)^T + alpha op(B) op(A)^T + beta C
+    (trans='T': A, B row-major (n, k), i.e. column-major k x n; trans='N': A, B row-major (k, n), i.e.
+    column-major n x k).  Honours the handle's math mode like ssfrk."""
+    _sfr2k(_ssfr2k_C, torch.float32, A, B, C, alpha, beta, transr, uplo, trans, n, k, lda, ldb)
+
+
+def dsfr2k(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, *, alpha: float = 1.0, beta: float = 0.0,
+           transr: str = "T", uplo: str = "U", trans: str = "T", n: int = None, k: int = None, lda: int = None,
+           ldb: int = None) -> None:
+    """FP64 twin of ssfr2k."""
+    _sfr2k(_dsfr2k_C, torch.float64, A, B, C, alpha, beta, transr, uplo, trans, n, k, lda, ldb)
+
+
+def dsfr_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, x: torch.Tensor, incx: int,
+             arf: torch.Tensor) -> None:
+    """FP64 twin of ssfr_raw."""
+    _validate_strided(x, "x", torch.float64)
+    _validate(arf, "arf", torch.float64)
+    _dsfr_C(handle, transr, uplo, n, float(alpha), x.data_ptr(), incx, arf.data_ptr())
+
+
+def dsfr2_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, x: torch.Tensor, incx: int,
+              y: torch.Tensor, incy: int, arf: torch.Tensor) -> None:
+    """FP64 twin of ssfr2_raw."""
+    _validate_strided(x, "x", torch.float64)
+    _validate_strided(y, "y", torch.float64)
+    _validate(arf, "arf", torch.float64)
+    _dsfr2_C(handle, transr, uplo, n, float(alpha), x.data_ptr(), incx, y.data_ptr(), incy, arf.data_ptr())
+
+
+def dsfr2k_raw(handle: Handle, transr: int, uplo: int, trans: int, n: int, k: int, alpha: float,
+               A: torch.Tensor, lda: int, B: torch.Tensor, ldb: int, beta: float, C: torch.Tensor) -> None:
+    """FP64 twin of ssfr2k_raw."""
+    _validate_strided(A, "A", torch.float64)
+    _validate_strided(B, "B", torch.float64)
+    _validate(C, "C", torch.float64)
+    _dsfr2k_C(handle, transr, uplo, trans, n, k, float(alpha), A.data_ptr(), lda, B.data_ptr(), ldb, float(beta),
+              C.data_ptr())
+
+
+def ssfr2k_raw(handle: Handle, transr: int, uplo: int, trans: int, n: int, k: int, alpha: float,
+               A: torch.Tensor, lda: int, B: torch.Tensor, ldb: int, beta: float, C: torch.Tensor) -> None:
+    """Low-level ssfr2k: integer enums, all dimensions explicit."""
+    _validate_strided(A, "A", torch.float32)
+    _validate_strided(B, "B", torch.float32)
+    _validate(C, "C")
+    _ssfr2k_C(handle, transr, uplo, trans, n, k, float(alpha), A.data_ptr(), lda, B.data_ptr(), ldb, float(beta),
+              C.data_ptr())
+
+
+# ---------------------------------------------------------------------------
 # Inverse from the RFP Cholesky factor — curfp/__init__.py:392-404, :661-690
 # ---------------------------------------------------------------------------
 def spftri_raw(handle: Handle, transr: int, uplo: int, n: int, A: torch.Tensor) -> None:
@@ -693,8 +847,7 @@ def _not_built(name):
     return fn
 
 
-for _name in ("ssfmv", "spfcon", "ssfr", "ssfr2", "ssfr2k", "spftrf_rcond",
-              "dsfmv", "dpfcon", "dsfr", "dsfr2", "dsfr2k", "dpftrf_rcond"):
+for _name in ("ssfmv", "spfcon", "spftrf_rcond", "dsfmv", "dpfcon", "dpftrf_rcond"):
     globals()[_name] = _not_built(_name)
     globals()[_name + "_raw"] = _not_built(_name + "_raw")
 del _name

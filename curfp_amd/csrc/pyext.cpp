@@ -195,6 +195,32 @@ PYBIND11_MODULE(_curfp_C, m)
     m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
         check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
     });
+    m.def("ssfr", [](Handle &h, int transr, int uplo, int n, float alpha, int64_t x, int incx, int64_t arf) {
+        check_status(curfpSsfr(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const float>(x), incx, P<float>(arf)));
+    });
+    m.def("dsfr", [](Handle &h, int transr, int uplo, int n, double alpha, int64_t x, int incx, int64_t arf) {
+        check_status(curfpDsfr(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const double>(x), incx, P<double>(arf)));
+    });
+    m.def("ssfr2", [](Handle &h, int transr, int uplo, int n, float alpha, int64_t x, int incx, int64_t y, int incy,
+                      int64_t arf) {
+        check_status(curfpSsfr2(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const float>(x), incx,
+                                P<const float>(y), incy, P<float>(arf)));
+    });
+    m.def("dsfr2", [](Handle &h, int transr, int uplo, int n, double alpha, int64_t x, int incx, int64_t y, int incy,
+                      int64_t arf) {
+        check_status(curfpDsfr2(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const double>(x), incx,
+                                P<const double>(y), incy, P<double>(arf)));
+    });
+    m.def("ssfr2k", [](Handle &h, int transr, int uplo, int trans, int n, int k, float alpha, int64_t A, int lda,
+                       int64_t B, int ldb, float beta, int64_t C) {
+        check_status(curfpSsfr2k(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, &alpha, P<const float>(A), lda,
+                                 P<const float>(B), ldb, &beta, P<float>(C)));
+    });
+    m.def("dsfr2k", [](Handle &h, int transr, int uplo, int trans, int n, int k, double alpha, int64_t A, int lda,
+                       int64_t B, int ldb, double beta, int64_t C) {
+        check_status(curfpDsfr2k(h.get(), OPS(transr), FILL(uplo), OPS(trans), n, k, &alpha, P<const double>(A), lda,
+                                 P<const double>(B), ldb, &beta, P<double>(C)));
+    });
     m.def("spftri", [](Handle &h, int transr, int uplo, int n, int64_t arf) {
         check_status(curfpSpftri(h.get(), OPS(transr), FILL(uplo), n, P<float>(arf)));
     });

@@ -372,6 +372,96 @@ static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
 }
 
 /* ------------------------------------------------------------------------ */
+/* ssfr / ssfr2 / ssfr2k (and D)                                             */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * C := alpha (X Y^T + Y X^T) + beta C on the RFP views (Y == nullptr: C :=
+ * alpha X X^T + beta C), with X, Y n x k given by their row views (row i of
+ * X at xv.sub(i, 0)).  Reference src/curfp_ssfr2k.cpp:121, src/curfp_ssfr2.cpp:94,
+ * src/curfp_ssfr.cpp:99: per block of the RFP matrix (v11 lower, v21 full,
+ * v22 lower) one or two GEMMs, the second accumulating onto the first.
+ * alpha == 0 does not read X or Y; beta == 0 does not read C.
+ */
+template <typename T>
+static curfpStatus_t sfr2k_core(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int64_t n,
+                                int64_t k, T a, const T *X, RfpView xv, const T *Y, RfpView yv, T b, T *C)
+{
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if ((a == T(0) || k == 0) && b == T(1)) return CURFP_STATUS_SUCCESS;
+    if (!C) return CURFP_STATUS_INVALID_VALUE;
+    const int64_t nt = n * (n + 1) / 2;
+    if ((a == T(0) || k == 0) && b == T(0)) return st_of(launch_fill_zero(C, (size_t)nt * sizeof(T), h->stream));
+    const int64_t K = (a == T(0)) ? 0 : k;
+    const bool two = Y != nullptr;
+    if (K > 0 && (!X || (two && !Y))) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const RfpView x1 = xv, x2 = xv.sub(g.n1, 0), y1 = yv, y2 = yv.sub(g.n1, 0);
+    const RfpMath m = Prec<T>::f64 ? RFP_MATH_F64 : math_of(h);
+    struct Blk { RfpView c, xr, xc, yr, yc; int64_t M, N; bool lower; } blk[3] = {
+        {g.v11, x1, x1, y1, y1, g.n1, g.n1, true},
+        {g.v21, x2, x1, y2, y1, g.n2, g.n1, false},
+        {g.v22, x2, x2, y2, y2, g.n2, g.n2, true},
+    };
+    for (const Blk &bk : blk) {
+        if (bk.M <= 0 || bk.N <= 0) continue;
+        GemmArgs ga = gemm_args();
+        ga.c = C; ga.vc = bk.c;
+        ga.M = bk.M; ga.N = bk.N; ga.K = K;
+        ga.lower = bk.lower;
+        /* X_rows Y_cols^T (or X_rows X_cols^T), then Y_rows X_cols^T */
+        ga.a = X; ga.va = bk.xr;
+        ga.b = two ? Y : X; ga.vb = two ? bk.yc : bk.xc;
+        ga.alpha = (double)a; ga.beta = (double)b;
+        hipError_t e = run_gemm<T>(h, ga, m);
+        if (e == hipSuccess && two && K > 0) {
+            ga.a = Y; ga.va = bk.yr;
+            ga.b = X; ga.vb = bk.xc;
+            ga.beta = 1.0;
+            e = run_gemm<T>(h, ga, m);
+        }
+        if (e != hipSuccess) return st_of(e);
+    }
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* row view of an n-vector with BLAS increment inc (inc < 0 walks backwards
+ * from element (n - 1) |inc|) as an n x 1 matrix */
+static inline RfpView vec_view(int64_t n, int inc)
+{
+    return inc > 0 ? RfpView{0, inc, 1} : RfpView{(n - 1) * (int64_t)(-inc), inc, 1};
+}
+
+template <typename T>
+static curfpStatus_t sfr_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const T *alpha,
+                              const T *x, int incx, const T *y, int incy, bool two, T *arf)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || !alpha || incx == 0 || (two && incy == 0)) return CURFP_STATUS_INVALID_VALUE;
+    return sfr2k_core<T>(h, transr, uplo, n, 1, *alpha, x, vec_view(n, incx), two ? y : nullptr,
+                         two ? vec_view(n, incy) : RfpView{0, 1, 1}, T(1), arf);
+}
+
+template <typename T>
+static curfpStatus_t sfr2k_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo,
+                                curfpOperation_t trans, int n, int k, const T *alpha, const T *A, int lda,
+                                const T *B, int ldb, const T *beta, T *C)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || k < 0 || lda < 1 || ldb < 1 || !alpha || !beta) return CURFP_STATUS_INVALID_VALUE;
+    if (trans != CURFP_OP_N && trans != CURFP_OP_T) return CURFP_STATUS_INVALID_VALUE;
+    if (trans == CURFP_OP_N ? (n > 0 && (lda < n || ldb < n)) : (k > 0 && (lda < k || ldb < k)))
+        return CURFP_STATUS_INVALID_VALUE;
+    /* row i of op(A) (n x k): trans = N -> A[i + p*lda], trans = T -> A[p + i*lda] */
+    const RfpView av = trans == CURFP_OP_N ? RfpView{0, 1, lda} : RfpView{0, lda, 1};
+    const RfpView bv = trans == CURFP_OP_N ? RfpView{0, 1, ldb} : RfpView{0, ldb, 1};
+    return sfr2k_core<T>(h, transr, uplo, n, k, *alpha, A, av, B, bv, *beta, C);
+}
+
+/* ------------------------------------------------------------------------ */
 /* spftrf                                                                    */
 /* ------------------------------------------------------------------------ */
 
@@ -1289,6 +1379,32 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curf
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
                                   const double *value, double *arf)
 { return add_diag_impl<double>(h, transr, uplo, n, value, arf); }
+
+curfpStatus_t curfpSsfr(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const float *alpha,
+                        const float *x, int incx, float *arf)
+{ return sfr_impl<float>(h, transr, uplo, n, alpha, x, incx, nullptr, 1, false, arf); }
+
+curfpStatus_t curfpDsfr(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const double *alpha,
+                        const double *x, int incx, double *arf)
+{ return sfr_impl<double>(h, transr, uplo, n, alpha, x, incx, nullptr, 1, false, arf); }
+
+curfpStatus_t curfpSsfr2(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const float *alpha,
+                         const float *x, int incx, const float *y, int incy, float *arf)
+{ return sfr_impl<float>(h, transr, uplo, n, alpha, x, incx, y, incy, true, arf); }
+
+curfpStatus_t curfpDsfr2(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const double *alpha,
+                         const double *x, int incx, const double *y, int incy, double *arf)
+{ return sfr_impl<double>(h, transr, uplo, n, alpha, x, incx, y, incy, true, arf); }
+
+curfpStatus_t curfpSsfr2k(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpOperation_t trans, int n,
+                          int k, const float *alpha, const float *A, int lda, const float *B, int ldb,
+                          const float *beta, float *C)
+{ return sfr2k_impl<float>(h, transr, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C); }
+
+curfpStatus_t curfpDsfr2k(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, curfpOperation_t trans, int n,
+                          int k, const double *alpha, const double *A, int lda, const double *B, int ldb,
+                          const double *beta, double *C)
+{ return sfr2k_impl<double>(h, transr, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C); }
 
 curfpStatus_t curfpSpftri(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, float *arf)
 { return pftri_impl<float>(h, transr, uplo, n, arf); }

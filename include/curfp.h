@@ -151,6 +151,29 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const float *value, float *arf);
 
+/* curfpSsfr — reference include/curfp.h:252-270, src/curfp_ssfr.cpp:99.
+ * C := alpha*x*x^T + C on the RFP matrix; x has n elements with BLAS increment
+ * incx (nonzero; negative walks backwards).  alpha is a host pointer. */
+curfpStatus_t curfpSsfr(curfpHandle_t handle, curfpOperation_t transr,
+                        curfpFillMode_t uplo, int n, const float *alpha,
+                        const float *x, int incx, float *arf);
+
+/* curfpSsfr2 — reference include/curfp.h:272-292, src/curfp_ssfr2.cpp:94.
+ * C := alpha*x*y^T + alpha*y*x^T + C. */
+curfpStatus_t curfpSsfr2(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, int n, const float *alpha,
+                         const float *x, int incx, const float *y, int incy,
+                         float *arf);
+
+/* curfpSsfr2k — reference include/curfp.h:294-319, src/curfp_ssfr2k.cpp:121.
+ * trans = N: C := alpha*A*B^T + alpha*B*A^T + beta*C (A, B n x k, lda, ldb >= n);
+ * trans = T: C := alpha*A^T*B + alpha*B^T*A + beta*C (A, B k x n, lda, ldb >= k).
+ * Column-major A, B; honours curfpSetMathMode like curfpSsfrk. */
+curfpStatus_t curfpSsfr2k(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, curfpOperation_t trans, int n, int k,
+                          const float *alpha, const float *A, int lda,
+                          const float *B, int ldb, const float *beta, float *C);
+
 /* curfpSpftri — reference include/curfp.h:175-186, src/curfp_spftri.cu:159-388.
  * Overwrites the RFP Cholesky factor from curfpSpftrf with inv(C) in the same
  * RFP layout (L^-1 in place, then L^-T L^-1).  Asynchronous on the handle
@@ -307,6 +330,20 @@ curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const double *value, double *arf);
+
+/* curfpDsfr / curfpDsfr2 / curfpDsfr2k — reference include/curfp.h:478-525.
+ * FP64 twins of curfpSsfr / curfpSsfr2 / curfpSsfr2k. */
+curfpStatus_t curfpDsfr(curfpHandle_t handle, curfpOperation_t transr,
+                        curfpFillMode_t uplo, int n, const double *alpha,
+                        const double *x, int incx, double *arf);
+curfpStatus_t curfpDsfr2(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, int n, const double *alpha,
+                         const double *x, int incx, const double *y, int incy,
+                         double *arf);
+curfpStatus_t curfpDsfr2k(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, curfpOperation_t trans, int n, int k,
+                          const double *alpha, const double *A, int lda,
+                          const double *B, int ldb, const double *beta, double *C);
 
 /* curfpDpftri — reference include/curfp.h:427-436.  FP64 twin of curfpSpftri. */
 curfpStatus_t curfpDpftri(curfpHandle_t handle, curfpOperation_t transr,

@@ -38,6 +38,14 @@ Functions and what pins each one (see tests/test_oracle.py):
         the dense product of an independently built symmetric matrix, the
         known answer of S = I, alpha = 0 / beta = 0 cases, and a planted
         one-triangle mistake.
+  sfr2k  — reference src/curfp_ssfr2k.cpp:121-231 (per view: SYR2K on v11
+        and v22, two GEMMs on v21); sfr / sfr2 — src/curfp_ssfr.cpp:99-172,
+        src/curfp_ssfr2.cpp:94-189 (the k = 1, beta = 1 cases).  Pinned by
+        sfr2k(A, A) = 2 sfrk(A) and the polarisation identity
+        x y^T + y x^T = ((x+y)(x+y)^T - (x-y)(x-y)^T) / 2 against the
+        LAPACK-pinned sfrk, by the dense product packed with trttf, the quick
+        returns of python_tests/test_ssfr2k.py:81-102, and a planted
+        halved-diagonal-block mistake.
   pftri  — reference src/curfp_spftri.cu:159-388 (inverse of the triangular
         factor, then L^-T L^-1); pinned against LAPACK ?pftri (scipy), an
         independent RFP routine, and C inv(C) = I.
@@ -55,7 +63,7 @@ from scipy.linalg import lapack as _lapack
 
 __all__ = [
     "views", "lower_index_map", "trttf", "tfttr", "diag_indices", "add_to_diagonal",
-    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "pftrf", "pftrs", "pftri", "lansf", "sfmm", "chain_flops",
+    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "sfr2k", "sfr", "sfr2", "pftrf", "pftrs", "pftri", "lansf", "sfmm", "chain_flops",
 ]
 
 
@@ -252,6 +260,49 @@ def sfrk(n, k, alpha, A, beta, C, transr="T", uplo="U", trans="N", exact=True):
             P = P + beta * _get(C, v, r, c).astype(acc)
         _put(C, v, r, c, P.astype(C.dtype), lower=lo)
     return C
+
+
+def sfr2k(n, k, alpha, A, B, beta, C, transr="T", uplo="U", trans="N"):
+    """C := alpha op(A) op(B)^T + alpha op(B) op(A)^T + beta C in RFP
+    (reference src/curfp_ssfr2k.cpp:121-231), float64 products rounded once
+    to C's dtype.  ``A``, ``B`` are the logical matrices: (n, k) for
+    trans='N', (k, n) for trans='T'.  Quick returns as sfrk (:137-148)."""
+    C = np.array(C, copy=True)
+    if n == 0 or ((alpha == 0 or k == 0) and beta == 1):
+        return C
+    if (alpha == 0 or k == 0) and beta == 0:
+        C[:] = 0
+        return C
+    n1, n2, _, v11, v21, v22 = views(n, transr, uplo)
+    if alpha == 0 or k == 0:
+        X = Y = np.zeros((n, 1))
+    else:
+        X = np.asarray(A, dtype=np.float64)
+        Y = np.asarray(B, dtype=np.float64)
+        if trans.upper() != "N":
+            X, Y = X.T, Y.T
+    X1, X2, Y1, Y2 = X[:n1], X[n1:], Y[:n1], Y[n1:]
+    for v, Xr, Xc, Yr, Yc, r, c, lo in ((v11, X1, X1, Y1, Y1, n1, n1, True), (v21, X2, X1, Y2, Y1, n2, n1, False),
+                                        (v22, X2, X2, Y2, Y2, n2, n2, True)):
+        if r == 0 or c == 0:
+            continue
+        P = alpha * (Xr @ Yc.T + Yr @ Xc.T)
+        if beta != 0:
+            P = P + beta * _get(C, v, r, c).astype(np.float64)
+        _put(C, v, r, c, P.astype(C.dtype), lower=lo)
+    return C
+
+
+def sfr(n, alpha, x, C, transr="T", uplo="U"):
+    """C := alpha x x^T + C (reference src/curfp_ssfr.cpp:99-172)."""
+    x = np.asarray(x, dtype=np.float64).reshape(n, 1)
+    return sfr2k(n, 1, alpha / 2.0, x, x, 1.0, C, transr, uplo, "N")
+
+
+def sfr2(n, alpha, x, y, C, transr="T", uplo="U"):
+    """C := alpha (x y^T + y x^T) + C (reference src/curfp_ssfr2.cpp:94-189)."""
+    return sfr2k(n, 1, alpha, np.asarray(x, dtype=np.float64).reshape(n, 1),
+                 np.asarray(y, dtype=np.float64).reshape(n, 1), 1.0, C, transr, uplo, "N")
 
 
 def _potrf_lower(T: np.ndarray):

@@ -333,3 +333,73 @@ def test_planted_pftri_mistake_is_caught(monkeypatch):
     real = O.rfp_lower_factor
     monkeypatch.setattr(O, "rfp_lower_factor", lambda a, m, transr="T", uplo="U": real(a, m, transr, uplo).T)
     assert np.abs(O.pftri(n, F, "N", "L") - ref).max() > 1e-6
+
+
+# --------------------------------------------------------------------------- rank-2k updates
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("n,k,trans", [(1, 1, "N"), (6, 2, "T"), (7, 3, "N"), (33, 9, "T"), (64, 5, "N")])
+def test_sfr2k_equals_twice_sfrk_for_equal_operands(t, u, n, k, trans):
+    """Small-integer A: sfr2k(A, A) = 2 sfrk(A) bit for bit (sfrk is pinned by
+    LAPACK ?sfrk golden vectors)."""
+    rng = np.random.default_rng(n + 10 * k)
+    A = rng.integers(-4, 5, size=(n, k) if trans == "N" else (k, n)).astype(np.float32)
+    C0 = rng.integers(-3, 4, size=n * (n + 1) // 2).astype(np.float32)
+    got = O.sfr2k(n, k, 1.5, A, A, 0.5, C0, t, u, trans)
+    ref = O.sfrk(n, k, 3.0, A, 0.5, C0, t, u, trans)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("n", [1, 2, 5, 16, 17, 65])
+def test_sfr2k_polarisation_against_sfrk(t, u, n):
+    """X Y^T + Y X^T = ((X+Y)(X+Y)^T - (X-Y)(X-Y)^T) / 2, both sides in RFP."""
+    k = 4
+    rng = np.random.default_rng(3 * n)
+    X, Y = rng.standard_normal((n, k)), rng.standard_normal((n, k))
+    C0 = rng.standard_normal(n * (n + 1) // 2)
+    got = O.sfr2k(n, k, -0.7, X, Y, 1.25, C0, t, u, "N")
+    P = O.sfrk(n, k, -0.35, X + Y, 1.25, C0, t, u, "N")
+    ref = O.sfrk(n, k, 0.35, X - Y, 1.0, P, t, u, "N")
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-12)
+    dense = -0.7 * (X @ Y.T + Y @ X.T) + 1.25 * O.rfp_to_dense_sym(C0, n, t, u)
+    assert np.allclose(got, O.trttf(dense, t, u), rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_sfr_and_sfr2_dense(t, u):
+    n = 11
+    rng = np.random.default_rng(4)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    C0 = rng.standard_normal(n * (n + 1) // 2)
+    S = O.rfp_to_dense_sym(C0, n, t, u)
+    assert np.allclose(O.sfr(n, 2.5, x, C0, t, u), O.trttf(S + 2.5 * np.outer(x, x), t, u), atol=1e-12)
+    assert np.allclose(O.sfr2(n, -1.5, x, y, C0, t, u),
+                       O.trttf(S - 1.5 * (np.outer(x, y) + np.outer(y, x)), t, u), atol=1e-12)
+
+
+def test_sfr2k_quick_returns():
+    """python_tests/test_ssfr2k.py:81-102: alpha = 0, beta = 1 passes C
+    through; alpha = beta = 0 zeroes C; neither reads A or B."""
+    C = np.arange(15, dtype=np.float32)
+    assert np.array_equal(O.sfr2k(5, 3, 0.0, None, None, 1.0, C), C)
+    assert np.array_equal(O.sfr2k(5, 0, 2.0, None, None, 1.0, C), C)
+    assert not O.sfr2k(5, 3, 0.0, None, None, 0.0, C).any()
+    assert np.array_equal(O.sfr2k(5, 3, 0.0, None, None, 2.0, C), 2 * C)
+
+
+def test_planted_sfr2k_mistake_is_caught(monkeypatch):
+    """Halving the diagonal blocks (adding the symmetric part of only one of
+    the two products there) must fail the dense check."""
+    n, k = 9, 3
+    rng = np.random.default_rng(8)
+    X, Y = rng.standard_normal((n, k)), rng.standard_normal((n, k))
+    C0 = np.zeros(n * (n + 1) // 2)
+    real_put = O._put
+
+    def put_half(arf, v, rows, cols, P, lower=False):
+        return real_put(arf, v, rows, cols, P / 2 if lower else P, lower)
+
+    monkeypatch.setattr(O, "_put", put_half)
+    got = O.sfr2k(n, k, 1.0, X, Y, 0.0, C0, "T", "U", "N")
+    assert not np.allclose(got, O.trttf(X @ Y.T + Y @ X.T, "T", "U"))
