@@ -318,6 +318,53 @@ def run_sfmm(curfp, torch, args, dev) -> int:
     return 0
 
 
+def run_sfr2k(curfp, torch, args, dev) -> int:
+    """ssfr2k C := A B^T + B A^T + C with row-major n x k A, B (trans T): 2 n^2 k
+    flops per call; the JSON line is the FP32 arithmetic (the library
+    default), the log also gives the BF16x3 and BF16 math modes."""
+    n, k = args.n, args.k
+    nt = n * (n + 1) // 2
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    A = torch.rand(n, k, device=dev, generator=g) * 2.0 - 1.0
+    B = torch.rand(n, k, device=dev, generator=g) * 2.0 - 1.0
+    C = torch.zeros(nt, device=dev)
+    steps = max(args.steps, 1)
+    fl = 2.0 * n * n * k
+    res = {}
+    for mode, cls in (("bf16", "gemm_bf16"), ("bf16x3", "gemm_bf16"), ("fp32", "gemm_f32")):
+        with curfp.math_mode(mode):
+            for _ in range(max(args.warmup, 1)):
+                curfp.ssfr2k(A, B, C, beta=1.0)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                curfp.ssfr2k(A, B, C, beta=1.0)
+            torch.cuda.synchronize()
+            wall = (time.perf_counter() - t0) / steps * 1e3
+            curfp.kernel_stats(reset=True)
+            curfp.set_profiling(True)
+            for _ in range(steps):
+                curfp.ssfr2k(A, B, C, beta=1.0)
+            st = curfp.kernel_stats(reset=True)[cls]
+            curfp.set_profiling(False)
+        kms = st["ms"] / steps
+        res[mode] = (wall, kms, fl / (wall * 1e-3) / 1e12, fl / (kms * 1e-3) / 1e12)
+        log(f"ssfr2k n={n} k={k} math={mode}: wall {wall:.2f} ms ({res[mode][2]:.1f} TFLOP/s), "
+            f"kernels {kms:.2f} ms ({res[mode][3]:.1f} TFLOP/s) in {st['launches'] // steps} launches")
+    wall, kms, tf_wall, tf = res["fp32"]
+    line = {"metric": "ssfr2k_tflops", "value": tf_wall, "unit": "TFLOP/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": wall, "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic",
+            "config": {"workload": f"ssfr2k trans T, n={n}, k={k}, FP32 math", "n": n, "k": k,
+                       "bf16_ms": res["bf16"][0], "bf16x3_ms": res["bf16x3"][0]},
+            "roofline": {"bound": "mfma", "achieved": tf, "peak": 157.3, "unit": "TFLOP/s", "frac": tf / 157.3,
+                         "traffic": None, "note": "six FP32 view GEMMs (two per RFP block), kernel time "
+                                                  "from HIP events summed over the launches of one call"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -328,9 +375,11 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri"], default="both",
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k"],
+                    default="both",
                     help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
-                         "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor")
+                         "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor; "
+                         "sfr2k: time ssfr2k (rank 2k, trans T) in the three math modes")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
@@ -355,6 +404,8 @@ def main() -> int:
         return run_norms(curfp, torch, args, dev)
     if args.mode == "sfmm":
         return run_sfmm(curfp, torch, args, dev)
+    if args.mode == "sfr2k":
+        return run_sfr2k(curfp, torch, args, dev)
     if args.mode == "pftri":
         return run_pftri(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam

@@ -65,6 +65,10 @@ try:
         dsfmm as _dsfmm_C,
         spftri as _spftri_C,
         dpftri as _dpftri_C,
+        ssfmv as _ssfmv_C,
+        dsfmv as _dsfmv_C,
+        spfcon as _spfcon_C,
+        dpfcon as _dpfcon_C,
         ssfr as _ssfr_C,
         dsfr as _dsfr_C,
         ssfr2 as _ssfr2_C,
@@ -523,6 +527,123 @@ def gen_residual(A, X, B, *, lam: float, alpha: float = 1.0, trans: str = "T", R
 
 
 # ---------------------------------------------------------------------------
+# Symmetric matrix-vector multiply — curfp/__init__.py:568-670
+# ---------------------------------------------------------------------------
+def ssfmv_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, arf: torch.Tensor, x: torch.Tensor,
+              incx: int, beta: float, y: torch.Tensor, incy: int) -> None:
+    """Low-level ssfmv: y := alpha A x + beta y (BLAS increments incx, incy)."""
+    _validate(arf, "arf")
+    _validate_strided(x, "x", torch.float32)
+    _validate_strided(y, "y", torch.float32)
+    _ssfmv_C(handle, transr, uplo, n, float(alpha), arf.data_ptr(), x.data_ptr(), incx, float(beta), y.data_ptr(),
+             incy)
+
+
+def dsfmv_raw(handle: Handle, transr: int, uplo: int, n: int, alpha: float, arf: torch.Tensor, x: torch.Tensor,
+              incx: int, beta: float, y: torch.Tensor, incy: int) -> None:
+    """FP64 twin of ssfmv_raw."""
+    _validate(arf, "arf", torch.float64)
+    _validate_strided(x, "x", torch.float64)
+    _validate_strided(y, "y", torch.float64)
+    _dsfmv_C(handle, transr, uplo, n, float(alpha), arf.data_ptr(), x.data_ptr(), incx, float(beta), y.data_ptr(),
+             incy)
+
+
+def _sfmv(fn, dtype, arf, x, y, alpha, beta, transr, uplo, n):
+    _validate(arf, "arf", dtype)
+    _vec(x, "x", dtype)
+    if n is None:
+        n = _n_from_rfp(arf.numel())
+    if y is None:
+        y = torch.zeros(n, dtype=dtype, device=arf.device)
+    _vec(y, "y", dtype)
+    if x.numel() != n:
+        raise ValueError(f"x has {x.numel()} elements but n={n}")
+    if y.numel() != n:
+        raise ValueError(f"y has {y.numel()} elements but n={n}")
+    fn(_get_handle(), _op(transr), _fill(uplo), n, float(alpha), arf.data_ptr(), x.data_ptr(), 1, float(beta),
+       y.data_ptr(), 1)
+    return y
+
+
+def ssfmv(arf: torch.Tensor, x: torch.Tensor, y: torch.Tensor = None, *, alpha: float = 1.0, beta: float = 0.0,
+          transr: str = "T", uplo: str = "U", n: int = None) -> torch.Tensor:
+    """y := alpha A x + beta y with A symmetric in RFP; y is created (zeros)
+    when not given and returned."""
+    return _sfmv(_ssfmv_C, torch.float32, arf, x, y, alpha, beta, transr, uplo, n)
+
+
+def dsfmv(arf: torch.Tensor, x: torch.Tensor, y: torch.Tensor = None, *, alpha: float = 1.0, beta: float = 0.0,
+          transr: str = "T", uplo: str = "U", n: int = None) -> torch.Tensor:
+    """FP64 twin of ssfmv."""
+    return _sfmv(_dsfmv_C, torch.float64, arf, x, y, alpha, beta, transr, uplo, n)
+
+
+# ---------------------------------------------------------------------------
+# Reciprocal condition estimate — curfp/__init__.py:844-955, 1739-1800
+# ---------------------------------------------------------------------------
+def spfcon_raw(handle: Handle, transr: int, uplo: int, n: int, C: torch.Tensor, anorm: float) -> float:
+    """Low-level spfcon: rcond = 1 / (||A^-1||_1 anorm) estimated from the RFP Cholesky factor C."""
+    _validate(C, "C")
+    return float(_spfcon_C(handle, transr, uplo, n, C.data_ptr(), float(anorm)))
+
+
+def dpfcon_raw(handle: Handle, transr: int, uplo: int, n: int, C: torch.Tensor, anorm: float) -> float:
+    """FP64 twin of spfcon_raw."""
+    _validate(C, "C", torch.float64)
+    return float(_dpfcon_C(handle, transr, uplo, n, C.data_ptr(), float(anorm)))
+
+
+def spfcon(C: torch.Tensor, anorm: float, *, transr: str = "T", uplo: str = "U", n: int = None) -> float:
+    """Estimate the reciprocal condition number 1 / (||A^-1||_1 anorm) from the
+    RFP Cholesky factor C (from spftrf, same transr / uplo); anorm is the
+    1-norm of A before factorisation (``slansf(C, '1')`` then).  Hager-Higham
+    estimator (LAPACK ?lacn2); 0.0 when anorm == 0."""
+    _validate(C, "C")
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    return float(_spfcon_C(_get_handle(), _op(transr), _fill(uplo), n, C.data_ptr(), float(anorm)))
+
+
+def dpfcon(C: torch.Tensor, anorm: float, *, transr: str = "T", uplo: str = "U", n: int = None) -> float:
+    """FP64 twin of spfcon."""
+    _validate(C, "C", torch.float64)
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    return float(_dpfcon_C(_get_handle(), _op(transr), _fill(uplo), n, C.data_ptr(), float(anorm)))
+
+
+def _pftrf_rcond(lansf, pftrf, pfcon, C, transr, uplo, n, check, name):
+    if n is None:
+        n = _n_from_rfp(C.numel())
+    h, t, u = _get_handle(), _op(transr), _fill(uplo)
+    anorm = float(lansf(h, NORM_ONE, t, u, n, C.data_ptr()))
+    info = pftrf(h, t, u, n, C.data_ptr())
+    if check and info != 0:
+        raise torch.linalg.LinAlgError(f"{name}: matrix is not positive definite "
+                                       f"(leading minor of order {info} is not positive definite)")
+    if info != 0 or anorm == 0.0:
+        return 0.0
+    return float(pfcon(h, t, u, n, C.data_ptr(), anorm))
+
+
+def spftrf_rcond(C: torch.Tensor, *, transr: str = "T", uplo: str = "U", n: int = None,
+                 check: bool = True) -> float:
+    """slansf('1') + spftrf + spfcon: C is overwritten with its Cholesky factor
+    and the estimated reciprocal condition number is returned (0.0 if C is not
+    positive definite and check is False; LinAlgError if check is True)."""
+    _validate(C, "C")
+    return _pftrf_rcond(_slansf_C, _spftrf_C, _spfcon_C, C, transr, uplo, n, check, "spftrf_rcond")
+
+
+def dpftrf_rcond(C: torch.Tensor, *, transr: str = "T", uplo: str = "U", n: int = None,
+                 check: bool = True) -> float:
+    """FP64 twin of spftrf_rcond."""
+    _validate(C, "C", torch.float64)
+    return _pftrf_rcond(_dlansf_C, _dpftrf_C, _dpfcon_C, C, transr, uplo, n, check, "dpftrf_rcond")
+
+
+# ---------------------------------------------------------------------------
 # Rank-1 / rank-2 / rank-2k updates — curfp/__init__.py:966-1200
 # ---------------------------------------------------------------------------
 def _vec(t: torch.Tensor, name: str, dtype) -> None:
@@ -834,20 +955,3 @@ def dlansf(C: torch.Tensor, norm: str = "1", *, transr: str = "T", uplo: str = "
     if n is None:
         n = _n_from_rfp(C.numel())
     return float(_dlansf_C(_get_handle(), _norm(norm), _op(transr), _fill(uplo), n, C.data_ptr()))
-
-
-# ---------------------------------------------------------------------------
-# Reference names outside the RFP hot path (curfp/__init__.py:42-80)
-# ---------------------------------------------------------------------------
-def _not_built(name):
-    def fn(*args, **kwargs):
-        raise NotImplementedError(f"curfp_amd: {name} is outside the RFP hot path built for MI355X")
-
-    fn.__name__ = name
-    return fn
-
-
-for _name in ("ssfmv", "spfcon", "spftrf_rcond", "dsfmv", "dpfcon", "dpftrf_rcond"):
-    globals()[_name] = _not_built(_name)
-    globals()[_name + "_raw"] = _not_built(_name + "_raw")
-del _name

@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 BUILD = os.path.join(HERE, "_objs")
 
-HIP_SOURCES = ["gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "lansf.hip", "rfp_drivers.hip"]
+HIP_SOURCES = ["gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "lansf.hip", "pfcon.hip", "rfp_drivers.hip"]
 LIB = os.path.join(HERE, "libcurfp_amd.so")
 
 HIPCC_FLAGS = [

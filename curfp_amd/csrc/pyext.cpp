@@ -195,6 +195,26 @@ PYBIND11_MODULE(_curfp_C, m)
     m.def("dadd_to_diagonal", [](Handle &h, int transr, int uplo, int n, double value, int64_t arf) {
         check_status(curfpDaddToDiagonal(h.get(), OPS(transr), FILL(uplo), n, &value, P<double>(arf)));
     });
+    m.def("ssfmv", [](Handle &h, int transr, int uplo, int n, float alpha, int64_t arf, int64_t x, int incx,
+                      float beta, int64_t y, int incy) {
+        check_status(curfpSsfmv(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const float>(arf), P<const float>(x),
+                                incx, &beta, P<float>(y), incy));
+    });
+    m.def("dsfmv", [](Handle &h, int transr, int uplo, int n, double alpha, int64_t arf, int64_t x, int incx,
+                      double beta, int64_t y, int incy) {
+        check_status(curfpDsfmv(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const double>(arf),
+                                P<const double>(x), incx, &beta, P<double>(y), incy));
+    });
+    m.def("spfcon", [](Handle &h, int transr, int uplo, int n, int64_t arf, float anorm) {
+        float r = 0.f;
+        check_status(curfpSpfcon(h.get(), OPS(transr), FILL(uplo), n, P<const float>(arf), anorm, &r));
+        return r;
+    });
+    m.def("dpfcon", [](Handle &h, int transr, int uplo, int n, int64_t arf, double anorm) {
+        double r = 0.0;
+        check_status(curfpDpfcon(h.get(), OPS(transr), FILL(uplo), n, P<const double>(arf), anorm, &r));
+        return r;
+    });
     m.def("ssfr", [](Handle &h, int transr, int uplo, int n, float alpha, int64_t x, int incx, int64_t arf) {
         check_status(curfpSsfr(h.get(), OPS(transr), FILL(uplo), n, &alpha, P<const float>(x), incx, P<float>(arf)));
     });

@@ -808,6 +808,107 @@ static curfpStatus_t pftrs_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
 }
 
 /* ------------------------------------------------------------------------ */
+/* spfcon: reciprocal condition estimate from the RFP Cholesky factor        */
+/* ------------------------------------------------------------------------ */
+
+static hipError_t pfcon_fill(float *x, int64_t n, int mode, int64_t j, hipStream_t s)
+{ return launch_pfcon_fill_f32(x, n, mode, j, s); }
+static hipError_t pfcon_fill(double *x, int64_t n, int mode, int64_t j, hipStream_t s)
+{ return launch_pfcon_fill_f64(x, n, mode, j, s); }
+static hipError_t pfcon_sign(float *x, float *g, int64_t n, hipStream_t s) { return launch_pfcon_sign_f32(x, g, n, s); }
+static hipError_t pfcon_sign(double *x, double *g, int64_t n, hipStream_t s) { return launch_pfcon_sign_f64(x, g, n, s); }
+static hipError_t pfcon_stats(const float *x, const float *g, int64_t n, int64_t j, PfconStats *o, hipStream_t s)
+{ return launch_pfcon_stats_f32(x, g, n, j, o, s); }
+static hipError_t pfcon_stats(const double *x, const double *g, int64_t n, int64_t j, PfconStats *o, hipStream_t s)
+{ return launch_pfcon_stats_f64(x, g, n, j, o, s); }
+
+/*
+ * Hager-Higham 1-norm estimate of inv(A) (LAPACK ?lacn2 with A^-T = A^-1, at
+ * most 5 iterations, the decision sequence of reference
+ * src/curfp_spfcon.cu:139-250), then rcond = (1 / est) / anorm.  Every
+ * product inv(A) x is one forward + backward substitution on the device
+ * iterate; the leaf inverses (and, in FP32, the sweep's look-back products)
+ * are built by the first solve and reused by the rest.
+ */
+template <typename T>
+static curfpStatus_t pfcon_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                const T *arf, T anorm, T *rcond)
+{
+    RFP_CHECK_HANDLE(h);
+    if (!rcond) return CURFP_STATUS_INVALID_VALUE;
+    *rcond = T(0);
+    if (n == 0 || anorm == T(0)) return CURFP_STATUS_SUCCESS;
+    if (n < 0 || !arf) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const size_t inv_bytes = (pftrs_scratch_bytes(g, Prec<T>::f64) + 255) & ~(size_t)255;
+    const size_t vec_bytes = ((size_t)n * sizeof(T) + 255) & ~(size_t)255;
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, inv_bytes + 2 * vec_bytes + 256, &ws);
+    if (e != hipSuccess) return st_of(e);
+    T *inv = (T *)ws;
+    T *x = (T *)((char *)ws + inv_bytes);
+    T *isgn = (T *)((char *)x + vec_bytes);
+    PfconStats *dst = (PfconStats *)((char *)isgn + vec_bytes);
+    hipStream_t s = h->stream;
+    bool prepared = false;
+    auto solve = [&]() -> hipError_t {
+        RFP_TRY(pftrs_core<T>(h, g, arf, x, RfpView{0, 1, n}, 1, inv, prepared));
+        prepared = true;
+        return hipSuccess;
+    };
+    PfconStats st;
+    auto stats = [&](bool dot, int64_t j) -> hipError_t {
+        RFP_TRY(pfcon_stats(x, dot ? isgn : nullptr, n, j, dst, s));
+        RFP_TRY(hipMemcpyAsync(&st, dst, sizeof(st), hipMemcpyDeviceToHost, s));
+        return hipStreamSynchronize(s);
+    };
+    constexpr int ITMAX = 5;
+    auto run = [&](T &est) -> hipError_t {
+        /* x = inv(A) (1/n, ..., 1/n) */
+        RFP_TRY(pfcon_fill(x, n, 0, 0, s));
+        RFP_TRY(solve());
+        RFP_TRY(stats(false, -1));
+        est = (T)st.asum;
+        if (n == 1) return hipSuccess;
+        /* x = inv(A) sign(x) */
+        RFP_TRY(pfcon_sign(x, isgn, n, s));
+        RFP_TRY(solve());
+        RFP_TRY(stats(false, -1));
+        int64_t j = st.imax;
+        int iter = 2;
+        for (;;) {
+            /* x = inv(A) e_j */
+            RFP_TRY(pfcon_fill(x, n, 1, j, s));
+            RFP_TRY(solve());
+            RFP_TRY(stats(true, -1));
+            est = (T)st.asum;
+            if (st.dot == st.asum) break;               /* sign pattern repeated */
+            RFP_TRY(pfcon_sign(x, isgn, n, s));
+            RFP_TRY(solve());
+            RFP_TRY(stats(false, j));
+            if (st.xj == st.xmax || iter >= ITMAX) break;
+            j = st.imax;
+            ++iter;
+        }
+        /* alternating-sign vector: est = max(est, 2 |inv(A) x|_1 / (3 n)) */
+        RFP_TRY(pfcon_fill(x, n, 2, 0, s));
+        RFP_TRY(solve());
+        RFP_TRY(stats(false, -1));
+        const T temp = (T)(2.0 * st.asum / (3.0 * (double)n));
+        if (temp > est) est = temp;
+        return hipSuccess;
+    };
+    T est = T(0);
+    e = run(est);
+    if (e != hipSuccess) return st_of(e);
+    if (est > T(0)) *rcond = (T(1) / est) / anorm;
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
 /* pack / unpack / diagonal                                                  */
 /* ------------------------------------------------------------------------ */
 
@@ -1083,6 +1184,33 @@ static curfpStatus_t sfmm_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
             return launch_sfmm_f64((const double *)arf, g, (const double *)B, yv, (double *)C, cv, N,
                                    (double)*alpha, (double)*beta, h->stream);
         return launch_sfmm_f32((const float *)arf, g, (const float *)B, yv, (float *)C, cv, N, (float)*alpha,
+                               (float)*beta, h->stream);
+    }));
+}
+
+/* y := alpha A x + beta y with BLAS increments: the n x 1 block product of the
+ * symmetric multiply, x and y addressed as row views (element i at
+ * i * inc, or (n - 1 - i) |inc| for inc < 0). */
+template <typename T>
+static curfpStatus_t sfmv_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const T *alpha,
+                               const T *arf, const T *x, int incx, const T *beta, T *y, int incy)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || !alpha || !beta || incx == 0 || incy == 0) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0) return CURFP_STATUS_SUCCESS;
+    if (!y || (*alpha != T(0) && (!arf || !x))) return CURFP_STATUS_INVALID_VALUE;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    const RfpView xv = vec_view(n, incx), yv = vec_view(n, incy);
+    const double fl = 2.0 * (double)n * (double)n;
+    const double by = (double)n * (n + 1) / 2.0 * sizeof(T) + 3.0 * (double)n * sizeof(T);
+    return st_of(timed(h, Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64 : CURFP_KERNEL_GEMM_F32, fl, by, [&] {
+        if (Prec<T>::f64)
+            return launch_sfmm_f64((const double *)arf, g, (const double *)x, xv, (double *)y, yv, 1,
+                                   (double)*alpha, (double)*beta, h->stream);
+        return launch_sfmm_f32((const float *)arf, g, (const float *)x, xv, (float *)y, yv, 1, (float)*alpha,
                                (float)*beta, h->stream);
     }));
 }
@@ -1405,6 +1533,22 @@ curfpStatus_t curfpDsfr2k(curfpHandle_t h, curfpOperation_t transr, curfpFillMod
                           int k, const double *alpha, const double *A, int lda, const double *B, int ldb,
                           const double *beta, double *C)
 { return sfr2k_impl<double>(h, transr, uplo, trans, n, k, alpha, A, lda, B, ldb, beta, C); }
+
+curfpStatus_t curfpSsfmv(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const float *alpha,
+                         const float *arf, const float *x, int incx, const float *beta, float *y, int incy)
+{ return sfmv_impl<float>(h, transr, uplo, n, alpha, arf, x, incx, beta, y, incy); }
+
+curfpStatus_t curfpDsfmv(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const double *alpha,
+                         const double *arf, const double *x, int incx, const double *beta, double *y, int incy)
+{ return sfmv_impl<double>(h, transr, uplo, n, alpha, arf, x, incx, beta, y, incy); }
+
+curfpStatus_t curfpSpfcon(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const float *arf,
+                          float anorm, float *rcond)
+{ return pfcon_impl<float>(h, transr, uplo, n, arf, anorm, rcond); }
+
+curfpStatus_t curfpDpfcon(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const double *arf,
+                          double anorm, double *rcond)
+{ return pfcon_impl<double>(h, transr, uplo, n, arf, anorm, rcond); }
 
 curfpStatus_t curfpSpftri(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, float *arf)
 { return pftri_impl<float>(h, transr, uplo, n, arf); }

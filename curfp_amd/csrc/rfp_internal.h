@@ -319,3 +319,26 @@ hipError_t launch_zero_f64(double *p, int n, hipStream_t s);
 size_t lansf_scratch_bytes(const RfpGeom &g, int norm, size_t elem);
 hipError_t launch_lansf_f32(const RfpGeom &g, int norm, const float *arf, void *ws, hipStream_t s);
 hipError_t launch_lansf_f64(const RfpGeom &g, int norm, const double *arf, void *ws, hipStream_t s);
+
+/*
+ * Vector kernels of the condition estimate (pfcon.hip).  fill: mode 0 x = 1/n,
+ * mode 1 x = e_j (0-based j), mode 2 x_i = (-1)^i (1 + i/(n-1)).  sign:
+ * x_i = isgn_i = (x_i >= 0 ? 1 : -1).  stats: one workgroup writes *out
+ * (isgn may be null: dot = 0; j < 0: xj = 0).
+ */
+struct PfconStats {
+    double asum;   /* sum |x_i| */
+    double dot;    /* sum isgn_i x_i */
+    double xmax;   /* max |x_i| */
+    double xj;     /* |x_j| */
+    int64_t imax;  /* first index of max |x_i| */
+    int64_t pad;
+};
+hipError_t launch_pfcon_fill_f32(float *x, int64_t n, int mode, int64_t j, hipStream_t s);
+hipError_t launch_pfcon_fill_f64(double *x, int64_t n, int mode, int64_t j, hipStream_t s);
+hipError_t launch_pfcon_sign_f32(float *x, float *isgn, int64_t n, hipStream_t s);
+hipError_t launch_pfcon_sign_f64(double *x, double *isgn, int64_t n, hipStream_t s);
+hipError_t launch_pfcon_stats_f32(const float *x, const float *isgn, int64_t n, int64_t j, PfconStats *out,
+                                  hipStream_t s);
+hipError_t launch_pfcon_stats_f64(const double *x, const double *isgn, int64_t n, int64_t j, PfconStats *out,
+                                  hipStream_t s);

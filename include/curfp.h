@@ -151,6 +151,24 @@ curfpStatus_t curfpSaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const float *value, float *arf);
 
+/* curfpSsfmv — reference include/curfp.h:151-172, src/curfp_ssfmv.cpp:134.
+ * y := alpha*A*x + beta*y with A symmetric in RFP, x and y with BLAS
+ * increments incx, incy (nonzero; negative walks backwards).  alpha = 0 reads
+ * neither arf nor x; beta = 0 does not read y. */
+curfpStatus_t curfpSsfmv(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, int n, const float *alpha,
+                         const float *arf, const float *x, int incx,
+                         const float *beta, float *y, int incy);
+
+/* curfpSpfcon — reference include/curfp.h:229-250, src/curfp_spfcon.cu:84.
+ * Estimate of rcond = 1 / (||A^-1||_1 anorm) from the RFP Cholesky factor arf
+ * (Hager-Higham / LAPACK ?lacn2, at most 5 iterations); anorm = ||A||_1 of
+ * the matrix before factorisation.  *rcond (host) is 0 for n = 0 or
+ * anorm = 0.  Synchronises the handle's stream once per iteration. */
+curfpStatus_t curfpSpfcon(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, const float *arf,
+                          float anorm, float *rcond);
+
 /* curfpSsfr — reference include/curfp.h:252-270, src/curfp_ssfr.cpp:99.
  * C := alpha*x*x^T + C on the RFP matrix; x has n elements with BLAS increment
  * incx (nonzero; negative walks backwards).  alpha is a host pointer. */
@@ -330,6 +348,17 @@ curfpStatus_t curfpDstfttr(curfpHandle_t handle, curfpOperation_t transr,
 curfpStatus_t curfpDaddToDiagonal(curfpHandle_t handle, curfpOperation_t transr,
                                   curfpFillMode_t uplo, int n,
                                   const double *value, double *arf);
+
+/* curfpDsfmv — reference include/curfp.h:410-424.  FP64 twin of curfpSsfmv. */
+curfpStatus_t curfpDsfmv(curfpHandle_t handle, curfpOperation_t transr,
+                         curfpFillMode_t uplo, int n, const double *alpha,
+                         const double *arf, const double *x, int incx,
+                         const double *beta, double *y, int incy);
+
+/* curfpDpfcon — reference include/curfp.h:464-476.  FP64 twin of curfpSpfcon. */
+curfpStatus_t curfpDpfcon(curfpHandle_t handle, curfpOperation_t transr,
+                          curfpFillMode_t uplo, int n, const double *arf,
+                          double anorm, double *rcond);
 
 /* curfpDsfr / curfpDsfr2 / curfpDsfr2k — reference include/curfp.h:478-525.
  * FP64 twins of curfpSsfr / curfpSsfr2 / curfpSsfr2k. */

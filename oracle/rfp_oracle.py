@@ -49,6 +49,12 @@ Functions and what pins each one (see tests/test_oracle.py):
   pftri  — reference src/curfp_spftri.cu:159-388 (inverse of the triangular
         factor, then L^-T L^-1); pinned against LAPACK ?pftri (scipy), an
         independent RFP routine, and C inv(C) = I.
+  pfcon  — reference src/curfp_spfcon.cu:84-271 (LAPACK ?lacn2 estimate
+        of ||A^-1||_1 with A^-1 = A^-T); pinned against LAPACK ?pocon
+        (scipy) on the same factor, the exact 1-norm of the inverse (the
+        estimate never exceeds it and equals it for diagonal matrices), the
+        n = 1 known answer of python_tests/test_spfcon.py:107-117, and a
+        planted forward-solve-only mistake.
   lansf  — reference src/curfp_slansf.cu:273-435 (max |a|, largest absolute
         column sum, Frobenius norm of the symmetric matrix); pinned by
         LAPACK ?lange on the dense matrix (an independent routine), the
@@ -63,7 +69,7 @@ from scipy.linalg import lapack as _lapack
 
 __all__ = [
     "views", "lower_index_map", "trttf", "tfttr", "diag_indices", "add_to_diagonal",
-    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "sfr2k", "sfr", "sfr2", "pftrf", "pftrs", "pftri", "lansf", "sfmm", "chain_flops",
+    "rfp_to_dense_sym", "dense_sym_to_rfp", "sfrk", "sfr2k", "sfr", "sfr2", "pftrf", "pftrs", "pftri", "pfcon", "lansf", "sfmm", "chain_flops",
 ]
 
 
@@ -380,6 +386,46 @@ def pftrs(n, arf, B, transr="T", uplo="U"):
         b1 = trsm(1.0, L11, b1, side=0, lower=1, trans_a=1)
     X = np.concatenate([b1, b2], axis=0).astype(dt)
     return X[:, 0] if squeeze else X
+
+
+def pfcon(n: int, arf: np.ndarray, anorm: float, transr: str = "T", uplo: str = "U", itmax: int = 5) -> float:
+    """Reciprocal condition estimate 1 / (est ||A^-1||_1 * anorm) from the RFP
+    Cholesky factor (reference src/curfp_spfcon.cu:84-271): LAPACK ?lacn2's
+    Hager-Higham iteration with A^-T = A^-1 and the reference's decisions —
+    start at inv(A) 1/n, then inv(A) sign(x), then unit vectors e_j at the
+    largest |x_j| until the sign pattern repeats, the largest entry stays put
+    or itmax iterations have run, and finally max(est, 2 |inv(A) v|_1 / (3n))
+    for v_i = (-1)^i (1 + i/(n-1)).  Products inv(A) x are two float64
+    triangular solves with the dense factor."""
+    if n == 0 or anorm == 0:
+        return 0.0
+    L = rfp_lower_factor(np.asarray(arf, dtype=np.float64), n, transr, uplo)
+
+    def solve(v):
+        y, _ = _lapack.dtrtrs(L, v, lower=1)
+        z, _ = _lapack.dtrtrs(L, y, lower=1, trans=1)
+        return z
+
+    x = solve(np.full(n, 1.0 / n))
+    est = float(np.abs(x).sum())
+    if n > 1:
+        isgn = np.where(x >= 0, 1.0, -1.0)
+        x = solve(isgn.copy())
+        j, it = int(np.argmax(np.abs(x))), 2
+        while True:
+            x = solve(np.eye(n)[j])
+            est = float(np.abs(x).sum())
+            if np.all(isgn * x >= 0):          # sign pattern repeated
+                break
+            isgn = np.where(x >= 0, 1.0, -1.0)
+            x = solve(isgn.copy())
+            jn = int(np.argmax(np.abs(x)))
+            if abs(x[j]) == abs(x[jn]) or it >= itmax:
+                break
+            j, it = jn, it + 1
+        v = np.where(np.arange(n) % 2 == 0, 1.0, -1.0) * (1.0 + np.arange(n) / (n - 1))
+        est = max(est, 2.0 * float(np.abs(solve(v)).sum()) / (3.0 * n))
+    return (1.0 / est) / anorm if est > 0 else 0.0
 
 
 def pftri(n: int, arf: np.ndarray, transr: str = "T", uplo: str = "U") -> np.ndarray:

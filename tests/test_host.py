@@ -105,5 +105,22 @@ def test_python_geometry_matches_oracle():
         curfp._fill("Q")
     with pytest.raises(ValueError):
         curfp.ssfrk(torch.zeros(4, 2), torch.zeros(10))  # not a device tensor
-    with pytest.raises(NotImplementedError):
-        curfp.spfcon(None, 1.0)
+
+
+# the public functions of the reference's Python package (curfp/__init__.py)
+REFERENCE_API = [
+    "set_stream", "rfp_diag_indices", "add_to_diagonal", "ssfrk_raw", "spftrf_raw", "spftrs_raw", "spftri_raw",
+    "ssfrk", "spftrf", "spftrs", "ssfmv_raw", "ssfmv", "spftri", "strttf_raw", "stfttr_raw", "strttf", "stfttr",
+    "slansf_raw", "slansf", "spfcon_raw", "spfcon", "spftrf_rcond", "ssfr", "ssfr_raw", "ssfr2", "ssfr2_raw",
+    "ssfr2k", "ssfr2k_raw", "ssfmm", "ssfmm_raw", "dstrttf", "dstfttr", "dstrttf_raw", "dstfttr_raw", "dsfrk",
+    "dsfrk_raw", "dpftrf", "dpftrf_raw", "dpftrs", "dpftrs_raw", "dpftri", "dpftri_raw", "dsfmv", "dsfmv_raw",
+    "dlansf", "dlansf_raw", "dpfcon", "dpfcon_raw", "dpftrf_rcond", "dsfr", "dsfr_raw", "dsfr2", "dsfr2_raw",
+    "dsfr2k", "dsfr2k_raw", "dsfmm", "dsfmm_raw",
+]
+
+
+def test_reference_api_is_complete():
+    import curfp_amd as curfp
+
+    missing = [f for f in REFERENCE_API if not callable(getattr(curfp, f, None))]
+    assert not missing, missing

@@ -403,3 +403,62 @@ def test_planted_sfr2k_mistake_is_caught(monkeypatch):
     monkeypatch.setattr(O, "_put", put_half)
     got = O.sfr2k(n, k, 1.0, X, Y, 0.0, C0, "T", "U", "N")
     assert not np.allclose(got, O.trttf(X @ Y.T + Y @ X.T, "T", "U"))
+
+
+# --------------------------------------------------------------------------- condition estimate
+
+def _spd_cond(n, seed, cond):
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    A = Q @ np.diag(np.linspace(1.0, cond, n)) @ Q.T
+    return (A + A.T) / 2.0
+
+
+@pytest.mark.parametrize("n", [2, 3, 8, 33, 128])
+@pytest.mark.parametrize("cond", [10.0, 1e4])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_pfcon_matches_lapack_pocon(n, cond, t, u):
+    """LAPACK ?pocon (scipy) on the same matrix; the estimate is a lower bound
+    of ||A^-1||_1, so rcond is at least the exact value."""
+    from scipy.linalg import lapack
+
+    A = _spd_cond(n, 10 * n + int(cond), cond)
+    anorm = np.linalg.norm(A, 1)
+    F, info = lapack.dpftrf(n, O.trttf(A, t, u), transr=t, uplo=u)
+    assert info == 0
+    got = O.pfcon(n, F, anorm, t, u)
+    ref, info = lapack.dpocon(np.linalg.cholesky(A).T, anorm)
+    assert info == 0
+    assert abs(got - ref) <= 1e-10 * ref
+    exact = 1.0 / (np.linalg.norm(np.linalg.inv(A), 1) * anorm)
+    assert got >= exact * (1 - 1e-12)
+
+
+def test_pfcon_known_answers():
+    """Diagonal A: the estimate is exact, rcond = min(d) / max(d); n = 1
+    (python_tests/test_spfcon.py:107-117): rcond = 1; anorm = 0: 0."""
+    d = np.array([3.0, 0.5, 7.0, 2.0, 9.0])
+    arf = O.trttf(np.diag(np.sqrt(d)), "N", "L")
+    assert abs(O.pfcon(5, arf, d.max(), "N", "L") - d.min() / d.max()) <= 1e-15
+    assert abs(O.pfcon(1, np.array([np.sqrt(4.0)]), 4.0) - 1.0) <= 1e-15
+    assert O.pfcon(5, arf, 0.0, "N", "L") == 0.0
+    assert O.pfcon(0, np.zeros(0), 1.0) == 0.0
+
+
+def test_planted_pfcon_mistake_is_caught(monkeypatch):
+    """Applying inv(L) alone (a forgotten backward solve) must fail the
+    LAPACK comparison."""
+    from scipy.linalg import lapack
+
+    n = 16
+    A = _spd_cond(n, 3, 100.0)
+    anorm = np.linalg.norm(A, 1)
+    F, _ = lapack.dpftrf(n, O.trttf(A, "T", "U"), transr="T", uplo="U")
+    ref, _ = lapack.dpocon(np.linalg.cholesky(A).T, anorm)
+    real = O._lapack.dtrtrs
+
+    def forward_only(a, b, lower=0, trans=0, **kw):
+        return (b, 0) if trans else real(a, b, lower=lower, trans=trans, **kw)
+
+    monkeypatch.setattr(O._lapack, "dtrtrs", forward_only)
+    assert abs(O.pfcon(n, F, anorm, "T", "U") - ref) > 1e-3 * ref
