@@ -276,6 +276,62 @@ def run_pftri(curfp, torch, args, dev) -> int:
     return 0
 
 
+def run_pfcon(curfp, torch, args, dev) -> int:
+    """spfcon on the FP32 factor of C = A A^T + lam I (the chain's matrix):
+    wall time per estimate (at most 7 forward + backward substitutions with
+    one vector, the leaf inverses built once, one reduction and read-back per
+    iteration), and ssfmv y = C x on the same n: HBM bytes n(n+1)/2 * 4 per
+    product against the 8 TB/s roofline."""
+    n, k, lam = args.n, args.k, args.lam
+    A, _ = make_inputs(n, k, 1, args.seed, dev)
+    C = torch.empty(n * (n + 1) // 2, dtype=torch.float32, device=dev)
+    curfp.ssfrk(A, C)
+    curfp.add_to_diagonal(C, lam)
+    anorm = curfp.slansf(C, "1")
+    x = torch.rand(n, device=dev) * 2.0 - 1.0
+    y = torch.zeros(n, device=dev)
+    steps = max(args.steps, 1)
+    for _ in range(max(args.warmup, 1)):
+        curfp.ssfmv(C, x, y)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        curfp.ssfmv(C, x, y)
+    torch.cuda.synchronize()
+    mv_ms = (time.perf_counter() - t0) / steps * 1e3
+    curfp.kernel_stats(reset=True)
+    curfp.set_profiling(True)
+    for _ in range(steps):
+        curfp.ssfmv(C, x, y)
+    st = curfp.kernel_stats(reset=True)["gemm_f32"]
+    curfp.set_profiling(False)
+    mv_kms = st["ms"] / max(st["launches"], 1)
+    mv_gbs = n * (n + 1) / 2 * 4 / (mv_kms * 1e-3) / 1e9
+    log(f"ssfmv n={n}: wall {mv_ms:.3f} ms, kernel {mv_kms:.3f} ms, {mv_gbs:.0f} GB/s of the packed matrix")
+    if curfp.spftrf(C, check=False) != 0:
+        raise RuntimeError("factorisation failed")
+    for _ in range(max(args.warmup, 0)):
+        curfp.spfcon(C, anorm)
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        rc = curfp.spfcon(C, anorm)
+        times.append(time.perf_counter() - t0)
+    ms = sorted(times)[len(times) // 2] * 1e3
+    log(f"spfcon n={n}: {ms:.2f} ms per estimate, rcond {rc:.4e}")
+    line = {"metric": "spfcon_ms", "value": ms, "unit": "ms", "n_gpus": 1, "steps": steps, "warmup": args.warmup,
+            "ms_per_step": ms, "higher_is_better": False, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"spfcon on the FP32 factor of A A^T + lam I, n={n}", "n": n, "k": k,
+                       "lambda": lam, "rcond": rc, "ssfmv_ms": mv_ms},
+            "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": 8000.0, "unit": "GB/s", "frac": mv_gbs / 8000.0,
+                         "traffic": None, "note": "ssfmv kernel (HIP events) on the same matrix: n(n+1)/2 * 4 "
+                                                  "bytes per product; spfcon is a chain of single-vector solves "
+                                                  "and host decisions, timed by wall clock"}}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_sfmm(curfp, torch, args, dev) -> int:
     """ssfmm C := A B (side L) with A a random n-order RFP matrix and B a
     column-major n x nrhs block: 2 n^2 nrhs flops per call on the FP32 matrix
@@ -375,11 +431,12 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k"],
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k", "pfcon"],
                     default="both",
                     help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
                          "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor; "
-                         "sfr2k: time ssfr2k (rank 2k, trans T) in the three math modes")
+                         "sfr2k: time ssfr2k (rank 2k, trans T) in the three math modes; "
+                         "pfcon: time spfcon and ssfmv on the chain's matrix")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
@@ -406,6 +463,8 @@ def main() -> int:
         return run_sfmm(curfp, torch, args, dev)
     if args.mode == "sfr2k":
         return run_sfr2k(curfp, torch, args, dev)
+    if args.mode == "pfcon":
+        return run_pfcon(curfp, torch, args, dev)
     if args.mode == "pftri":
         return run_pftri(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
