@@ -243,6 +243,162 @@ __global__ __launch_bounds__(256) void colsum_max_kernel(const T *S, LansfGeom g
     if (threadIdx.x == 0) atomicMax(acc + 3, key_of(m));
 }
 
+/*
+ * y := alpha A x + beta y with A symmetric in RFP: the tiles of the 1-norm's
+ * traversal, each read once.  A tile L (rows of block bi, columns of block
+ * bj, bi >= bj) contributes L x_bj to the rows of bi (slot bj) and, off the
+ * diagonal, L^T x_bi to the columns of bj (slot bi); a diagonal tile writes
+ * L x + strict(L)^T x into slot bi.  Every (slot, index) pair of S is
+ * written by exactly one tile.  A thread's 16-byte piece runs along the
+ * contiguous direction: its products summed within the piece give one
+ * partial of the other direction (reduced over the RUN lanes that share the
+ * piece's line), and the products themselves accumulate over the thread's
+ * pieces for the VN lines of the contiguous direction (reduced over the
+ * 256 / RUN thread groups through LDS).
+ */
+template <typename T>
+__global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g, const T *x, RfpView xv, T *S)
+{
+    typedef typename Vec16<T>::type V;
+    constexpr int VN = Vec16<T>::N;
+    constexpr int PER = NT * NT / VN / 256;
+    constexpr int RUN = NT / VN;         /* lanes along one line */
+    constexpr int GROUPS = 256 / RUN;
+    __shared__ T xr[NT], xc[NT], line[NT], red[GROUPS][NT];
+    const int tid = threadIdx.x;
+    int64_t bi, bj;
+    tri_tile_coords((int64_t)blockIdx.x, g.nb, bi, bj);
+    int64_t gr, lr, gc, lc;
+    int mr, mc;
+    lansf_block(g, (int)bi, gr, lr, mr);
+    lansf_block(g, (int)bj, gc, lc, mc);
+    const RfpView v = bi < g.s1 ? g.v11.sub(lr, lc) : (bj < g.s1 ? g.v21.sub(lr, lc) : g.v22.sub(lr, lc));
+    const bool diag = bi == bj;
+    const bool rc = v.rs == 1;            /* rows contiguous (column view) */
+    const int a0 = (tid % RUN) * VN;      /* offset along the contiguous direction (fixed per thread) */
+    T acc[VN];
+#pragma unroll
+    for (int k = 0; k < VN; ++k) acc[k] = T(0);
+    const bool interior = !diag && mr == NT && mc == NT;
+    /* every load of the tile issued before any arithmetic */
+    T u_all[PER][VN];
+    if (interior) {
+        typedef T Vu __attribute__((ext_vector_type(VN), aligned(sizeof(T))));
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int b0 = tid / RUN + GROUPS * e;
+            const Vu w = *(const Vu *)(a + (rc ? v.at(a0, b0) : v.at(b0, a0)));
+#pragma unroll
+            for (int k = 0; k < VN; ++k) u_all[e][k] = w[k];
+        }
+    } else {
+#pragma unroll
+        for (int e = 0; e < PER; ++e) {
+            const int b0 = tid / RUN + GROUPS * e;
+#pragma unroll
+            for (int k = 0; k < VN; ++k) {
+                const int r = rc ? a0 + k : b0, c = rc ? b0 : a0 + k;
+                u_all[e][k] = (r < mr && c < mc && (!diag || r >= c)) ? a[v.at(r, c)] : T(0);
+            }
+        }
+    }
+    if (tid < NT) xr[tid] = tid < mr ? x[xv.off + (gr + tid) * xv.rs] : T(0);
+    else          xc[tid - NT] = tid - NT < mc ? x[xv.off + (gc + tid - NT) * xv.rs] : T(0);
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+        const int b0 = tid / RUN + GROUPS * e;     /* the line: a column (rc) or a row */
+        const T *u = u_all[e];
+        T part = 0;
+        if (rc) {
+            /* u[k] = L(a0 + k, b0): rows a0.. take L x; column b0 takes the
+             * strict L^T x */
+            const T xv_c = xc[b0];
+#pragma unroll
+            for (int k = 0; k < VN; ++k) {
+                acc[k] += u[k] * xv_c;
+                part += (diag && a0 + k == b0) ? T(0) : u[k] * xr[a0 + k];
+            }
+        } else {
+            /* u[k] = L(b0, a0 + k): row b0 takes L x; columns a0.. the strict L^T x */
+            const T xv_r = xr[b0];
+#pragma unroll
+            for (int k = 0; k < VN; ++k) {
+                part += u[k] * xc[a0 + k];
+                acc[k] += (diag && b0 == a0 + k) ? T(0) : u[k] * xv_r;
+            }
+        }
+#pragma unroll
+        for (int o = RUN / 2; o >= 1; o >>= 1) part += __shfl_xor(part, o);
+        if (tid % RUN == 0) line[b0] = part;
+    }
+#pragma unroll
+    for (int k = 0; k < VN; ++k) red[tid / RUN][a0 + k] = acc[k];
+    __syncthreads();
+    if (tid < NT) {
+        T s = 0;
+#pragma unroll
+        for (int q = 0; q < GROUPS; ++q) s += red[q][tid];
+        /* rc: s is the row sum of row tid, line[] the column sums; else the
+         * reverse */
+        const T rsum = rc ? s : line[tid], csum = rc ? line[tid] : s;
+        if (diag) {
+            if (tid < mr) S[bi * g.n + gr + tid] = rsum + csum;
+        } else {
+            if (tid < mr) S[bj * g.n + gr + tid] = rsum;
+            if (tid < mc) S[bi * g.n + gc + tid] = csum;
+        }
+    }
+}
+
+/* y(i) = alpha sum_slot S[slot][i] + beta y(i) (nslots = 0: alpha A x is 0;
+ * beta = 0: y is not read) */
+template <typename T>
+__global__ __launch_bounds__(256) void symv_finish_kernel(const T *S, int64_t n, int nslots, T alpha, T beta, T *y,
+                                                          RfpView yv)
+{
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+        double part[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) part[u] = 0.0;
+        int b = 0;
+        for (; b + 8 <= nslots; b += 8) {
+            T t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) t[u] = S[(int64_t)(b + u) * n + j];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) part[u] += (double)t[u];
+        }
+        for (; b < nslots; ++b) part[0] += (double)S[(int64_t)b * n + j];
+        double s = 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += part[u];
+        T *yp = y + yv.off + j * yv.rs;
+        const T val = alpha * (T)s;
+        *yp = beta != T(0) ? val + beta * *yp : val;
+    }
+}
+
+LansfGeom lansf_geom(const RfpGeom &r);
+
+template <typename T>
+hipError_t symv(const RfpGeom &rg, const T *arf, const T *x, RfpView xv, T alpha, T beta, T *y, RfpView yv, void *ws,
+                hipStream_t s)
+{
+    const LansfGeom g = lansf_geom(rg);
+    T *S = (T *)ws;
+    int nslots = 0;
+    if (alpha != T(0)) {
+        const int64_t ntiles = (int64_t)g.nb * (g.nb + 1) / 2;
+        hipLaunchKernelGGL((symv_tiles_kernel<T>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, x, xv, S);
+        nslots = g.nb;
+    }
+    const unsigned blocks = (unsigned)std::min<int64_t>((g.n + 255) / 256, 2048);
+    hipLaunchKernelGGL((symv_finish_kernel<T>), dim3(blocks), dim3(256), 0, s, (const T *)S, g.n, nslots, alpha, beta,
+                       y, yv);
+    return hipGetLastError();
+}
+
 __global__ void zero_u64_kernel(unsigned long long *p, int n)
 {
     if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
@@ -289,6 +445,19 @@ size_t lansf_scratch_bytes(const RfpGeom &rg, int norm, size_t elem)
     const LansfGeom g = lansf_geom(rg);
     return 256 + (norm == 1 ? (size_t)g.nb * (size_t)g.n * elem : 0);
 }
+
+size_t symv_scratch_bytes(const RfpGeom &rg, size_t elem)
+{
+    const LansfGeom g = lansf_geom(rg);
+    return (size_t)g.nb * (size_t)g.n * elem;
+}
+
+hipError_t launch_symv_f32(const RfpGeom &g, const float *arf, const float *x, RfpView xv, float alpha, float beta,
+                           float *y, RfpView yv, void *ws, hipStream_t s)
+{ return symv<float>(g, arf, x, xv, alpha, beta, y, yv, ws, s); }
+hipError_t launch_symv_f64(const RfpGeom &g, const double *arf, const double *x, RfpView xv, double alpha,
+                           double beta, double *y, RfpView yv, void *ws, hipStream_t s)
+{ return symv<double>(g, arf, x, xv, alpha, beta, y, yv, ws, s); }
 
 hipError_t launch_lansf_f32(const RfpGeom &g, int norm, const float *arf, void *ws, hipStream_t s)
 { return lansf<float>(g, norm, arf, ws, s); }

@@ -1188,9 +1188,9 @@ static curfpStatus_t sfmm_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
     }));
 }
 
-/* y := alpha A x + beta y with BLAS increments: the n x 1 block product of the
- * symmetric multiply, x and y addressed as row views (element i at
- * i * inc, or (n - 1 - i) |inc| for inc < 0). */
+/* y := alpha A x + beta y with BLAS increments (element i at i * inc, or
+ * (n - 1 - i) |inc| for inc < 0): each tile of A read once, its products
+ * with x into per-tile slots, then one pass summing the slots. */
 template <typename T>
 static curfpStatus_t sfmv_impl(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n, const T *alpha,
                                const T *arf, const T *x, int incx, const T *beta, T *y, int incy)
@@ -1204,14 +1204,17 @@ static curfpStatus_t sfmv_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
     if (!y || (*alpha != T(0) && (!arf || !x))) return CURFP_STATUS_INVALID_VALUE;
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
     const RfpView xv = vec_view(n, incx), yv = vec_view(n, incy);
+    void *ws = nullptr;
+    hipError_t e = rfp_workspace(h, symv_scratch_bytes(g, sizeof(T)), &ws);
+    if (e != hipSuccess) return st_of(e);
     const double fl = 2.0 * (double)n * (double)n;
     const double by = (double)n * (n + 1) / 2.0 * sizeof(T) + 3.0 * (double)n * sizeof(T);
-    return st_of(timed(h, Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64 : CURFP_KERNEL_GEMM_F32, fl, by, [&] {
+    return st_of(timed(h, CURFP_KERNEL_OTHER, fl, by, [&] {
         if (Prec<T>::f64)
-            return launch_sfmm_f64((const double *)arf, g, (const double *)x, xv, (double *)y, yv, 1,
-                                   (double)*alpha, (double)*beta, h->stream);
-        return launch_sfmm_f32((const float *)arf, g, (const float *)x, xv, (float *)y, yv, 1, (float)*alpha,
-                               (float)*beta, h->stream);
+            return launch_symv_f64(g, (const double *)arf, (const double *)x, xv, (double)*alpha, (double)*beta,
+                                   (double *)y, yv, ws, h->stream);
+        return launch_symv_f32(g, (const float *)arf, (const float *)x, xv, (float)*alpha, (float)*beta, (float *)y,
+                               yv, ws, h->stream);
     }));
 }
 

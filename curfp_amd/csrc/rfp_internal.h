@@ -319,6 +319,13 @@ hipError_t launch_zero_f64(double *p, int n, hipStream_t s);
 size_t lansf_scratch_bytes(const RfpGeom &g, int norm, size_t elem);
 hipError_t launch_lansf_f32(const RfpGeom &g, int norm, const float *arf, void *ws, hipStream_t s);
 hipError_t launch_lansf_f64(const RfpGeom &g, int norm, const double *arf, void *ws, hipStream_t s);
+/* y := alpha A x + beta y, A symmetric in RFP, x and y n-vectors as views
+ * (element i at off + i * rs); ws holds symv_scratch_bytes (lansf.hip). */
+size_t symv_scratch_bytes(const RfpGeom &g, size_t elem);
+hipError_t launch_symv_f32(const RfpGeom &g, const float *arf, const float *x, RfpView xv, float alpha, float beta,
+                           float *y, RfpView yv, void *ws, hipStream_t s);
+hipError_t launch_symv_f64(const RfpGeom &g, const double *arf, const double *x, RfpView xv, double alpha,
+                           double beta, double *y, RfpView yv, void *ws, hipStream_t s);
 
 /*
  * Vector kernels of the condition estimate (pfcon.hip).  fill: mode 0 x = 1/n,
