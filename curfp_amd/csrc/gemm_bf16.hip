@@ -460,6 +460,41 @@ hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
 
 }  // namespace
 
+namespace {
+
+/* out[i * ldo + p] = bf16(A(i, p)) for p < K, 0 for K <= p < Kp (and the
+ * split-BF16 low part bf16(A - hi) into lo when lo is given); A(i, p) at
+ * off + i * rs + p * cs.  Consecutive threads run along A's contiguous
+ * direction. */
+__global__ __launch_bounds__(256) void to_bf16_rows_kernel(const float *A, RfpView av, int64_t rows, int64_t K,
+                                                           int64_t Kp, int64_t ldo, unsigned short *hi,
+                                                           unsigned short *lo)
+{
+    const int64_t total = rows * Kp;
+    const bool along_k = av.cs == 1;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = along_k ? t / Kp : t % rows;
+        const int64_t p = along_k ? t % Kp : t / rows;
+        const float x = p < K ? A[av.off + i * av.rs + p * av.cs] : 0.f;
+        const unsigned short h = bf16_bits(x);
+        hi[i * ldo + p] = h;
+        if (lo) lo[i * ldo + p] = bf16_bits(x - to_f(h));
+    }
+}
+
+}  // namespace
+
+hipError_t launch_to_bf16_rows(const float *A, RfpView av, int64_t rows, int64_t K, int64_t Kp, int64_t ldo,
+                               unsigned short *hi, unsigned short *lo, hipStream_t s)
+{
+    if (rows <= 0 || Kp <= 0) return hipSuccess;
+    if (K > Kp || Kp > ldo) return hipErrorInvalidValue;
+    const int64_t total = rows * Kp;
+    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
+    hipLaunchKernelGGL(to_bf16_rows_kernel, dim3(blocks), dim3(256), 0, s, A, av, rows, K, Kp, ldo, hi, lo);
+    return hipGetLastError();
+}
+
 hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
 {
     if (g.M <= 0 || g.N <= 0) return hipSuccess;

@@ -399,8 +399,58 @@ static curfpStatus_t sfr2k_core(curfpHandle_t h, curfpOperation_t transr, curfpF
     const bool two = Y != nullptr;
     if (K > 0 && (!X || (two && !Y))) return CURFP_STATUS_INVALID_VALUE;
     const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
-    const RfpView x1 = xv, x2 = xv.sub(g.n1, 0), y1 = yv, y2 = yv.sub(g.n1, 0);
     const RfpMath m = Prec<T>::f64 ? RFP_MATH_F64 : math_of(h);
+    /* BF16 modes on a large update: op(A), op(B) converted once to BF16 row
+     * blocks P = [X | Y] and Q = [Y | X] (each half k padded to a multiple of
+     * 8 with zeros), so P Q^T = X Y^T + Y X^T is one GEMM of depth 2 Kp per
+     * view on the BF16 row-block kernels (direct global -> LDS loads): C is
+     * read and written once, and no operand tile is converted while staged */
+    const bool convert = !Prec<T>::f64 && m != RFP_MATH_F32 && K > 0 && n >= 2048 && K >= 32;
+    if (convert) {
+        const int64_t Kp = (K + 7) / 8 * 8, ld = 2 * Kp;
+        const bool x3 = m == RFP_MATH_BF16X3;
+        const size_t blk = ((size_t)n * ld * sizeof(unsigned short) + 255) & ~(size_t)255;
+        void *ws = nullptr;
+        hipError_t e = rfp_workspace(h, blk * (x3 ? 4 : 2), &ws);
+        if (e != hipSuccess) return st_of(e);
+        unsigned short *P = (unsigned short *)ws, *Q = (unsigned short *)((char *)ws + blk);
+        unsigned short *Pl = x3 ? (unsigned short *)((char *)ws + 2 * blk) : nullptr;
+        unsigned short *Ql = x3 ? (unsigned short *)((char *)ws + 3 * blk) : nullptr;
+        const T *Yo = two ? Y : X;
+        const RfpView yo = two ? yv : xv;
+        struct Cv { const T *src; RfpView v; unsigned short *hi, *lo; } cv[4] = {
+            {X, xv, P, Pl}, {Yo, yo, P + Kp, Pl ? Pl + Kp : nullptr},
+            {Yo, yo, Q, Ql}, {X, xv, Q + Kp, Ql ? Ql + Kp : nullptr},
+        };
+        for (const Cv &c : cv) {
+            e = timed(h, CURFP_KERNEL_PACK, 0.0, (double)n * K * 4.0 + (double)n * Kp * (x3 ? 4.0 : 2.0), [&] {
+                return launch_to_bf16_rows((const float *)c.src, c.v, n, K, Kp, ld, c.hi, c.lo, h->stream);
+            });
+            if (e != hipSuccess) return st_of(e);
+        }
+        struct Blk { RfpView c; int64_t r0, c0, M, N; bool lower; } blk3[3] = {
+            {g.v11, 0, 0, g.n1, g.n1, true},
+            {g.v21, g.n1, 0, g.n2, g.n1, false},
+            {g.v22, g.n1, g.n1, g.n2, g.n2, true},
+        };
+        for (const Blk &bk : blk3) {
+            if (bk.M <= 0 || bk.N <= 0) continue;
+            GemmArgs ga = gemm_args();
+            ga.c = C; ga.vc = bk.c;
+            ga.M = bk.M; ga.N = bk.N; ga.K = ld;
+            ga.lower = bk.lower;
+            ga.bf16_rows = true;
+            ga.a = P; ga.a_lo = Pl; ga.va = RfpView{bk.r0 * ld, ld, 1};
+            ga.b = Q; ga.b_lo = Ql; ga.vb = RfpView{bk.c0 * ld, ld, 1};
+            /* a rank-k update (no Y) is P = [X | X], Q = [X | X] at half alpha */
+            ga.alpha = two ? (double)a : 0.5 * (double)a;
+            ga.beta = (double)b;
+            e = run_gemm<T>(h, ga, m);
+            if (e != hipSuccess) return st_of(e);
+        }
+        return CURFP_STATUS_SUCCESS;
+    }
+    const RfpView x1 = xv, x2 = xv.sub(g.n1, 0), y1 = yv, y2 = yv.sub(g.n1, 0);
     struct Blk { RfpView c, xr, xc, yr, yc; int64_t M, N; bool lower; } blk[3] = {
         {g.v11, x1, x1, y1, y1, g.n1, g.n1, true},
         {g.v21, x2, x1, y2, y1, g.n2, g.n1, false},

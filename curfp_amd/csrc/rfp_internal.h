@@ -207,6 +207,11 @@ hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s);
  * requirements */
 hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s);
 hipError_t launch_gemm_f64(const GemmArgs &g, hipStream_t s);
+/* BF16 row blocks of an FP32 operand: hi[i * ldo + p] = bf16(A(i, p)) for
+ * p < K, zero for K <= p < Kp (Kp <= ldo); lo (optional) the split-BF16 low
+ * parts. */
+hipError_t launch_to_bf16_rows(const float *A, RfpView av, int64_t rows, int64_t K, int64_t Kp, int64_t ldo,
+                               unsigned short *hi, unsigned short *lo, hipStream_t s);
 
 /* Largest N (inplace = 1) / M (inplace = 2) the in-place GEMM supports. */
 int gemm_inplace_limit(bool f64);

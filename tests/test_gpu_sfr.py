@@ -234,3 +234,33 @@ def test_dsfr_family_matches_oracle(curfp, n, t, u):
     curfp.dsfr2k(torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda(), C, alpha=-0.6, beta=0.4, transr=t, uplo=u)
     ref = O.sfr2k(n, k, -0.6, A.T, B.T, 0.4, C0, t, u, "T")
     assert np.abs(C.cpu().numpy() - ref).max() <= 8 * (k + 2) * 2.0 ** -53 * (np.abs(ref).max() + 2 * k)
+
+
+@pytest.mark.parametrize("trans,k", [("T", 520), ("N", 77), ("T", 33)])
+@pytest.mark.parametrize("t,u", [("N", "L"), ("T", "U")])
+def test_ssfr2k_bf16_row_blocks(curfp, trans, k, t, u):
+    """n >= 2048 in the BF16 modes: op(A), op(B) converted once to BF16 row
+    blocks (k padded to a multiple of 8), the updates on the BF16 row-block
+    kernels.  BF16x3 within FP32 accuracy; BF16 equal to the float64 product
+    of the BF16-rounded operands to FP32 accumulation accuracy."""
+    n = 4100
+    rng = np.random.default_rng(k + ord(trans))
+    shape = (n, k) if trans == "T" else (k, n)      # row-major tensors
+    A = rng.standard_normal(shape).astype(np.float32)
+    B = rng.standard_normal(shape).astype(np.float32)
+    C0 = sym_rfp(n, rng, t, u, np.float32)
+    At, Bt = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    # the oracle takes the logical matrices: (k, n) for trans T, (n, k) for N — A.T of either tensor
+    ref = O.sfr2k(n, k, 0.5, A.T, B.T, 0.75, C0.astype(np.float64), t, u, trans)
+    C = torch.from_numpy(C0).cuda()
+    with curfp.math_mode("bf16x3"):
+        curfp.ssfr2k(At, Bt, C, alpha=0.5, beta=0.75, transr=t, uplo=u, trans=trans)
+    err = np.abs(C.cpu().numpy() - ref).max() / np.abs(ref).max()
+    assert err <= 1e-5, err
+    bf = lambda X: torch.from_numpy(X).to(torch.bfloat16).double().numpy()
+    ref16 = O.sfr2k(n, k, 0.5, bf(A).T, bf(B).T, 0.75, C0.astype(np.float64), t, u, trans)
+    C = torch.from_numpy(C0).cuda()
+    with curfp.math_mode("bf16"):
+        curfp.ssfr2k(At, Bt, C, alpha=0.5, beta=0.75, transr=t, uplo=u, trans=trans)
+    err = np.abs(C.cpu().numpy() - ref16).max() / np.abs(ref16).max()
+    assert err <= 1e-5, err
