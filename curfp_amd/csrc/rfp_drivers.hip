@@ -354,6 +354,39 @@ static curfpStatus_t sfrk_impl(curfpHandle_t h, curfpOperation_t transr, curfpFi
         {g.v21, a2, a1, g.n2, g.n1, false},
         {g.v22, a2, a2, g.n2, g.n2, true},
     };
+    /* FP32 A in a BF16 mode on a large update: op(A) converted once to BF16
+     * row blocks (k padded to a multiple of 8 with zeros) for the row-block
+     * kernels, instead of converting every operand tile as it is staged */
+    if (!a_bf16 && !Prec<T>::f64 && m != RFP_MATH_F32 && n >= 2048 && K >= 32) {
+        const int64_t Kp = (K + 7) / 8 * 8;
+        const bool x3 = m == RFP_MATH_BF16X3;
+        const size_t bytes = ((size_t)n * Kp * sizeof(unsigned short) + 255) & ~(size_t)255;
+        void *ws = nullptr;
+        hipError_t e = rfp_workspace(h, bytes * (x3 ? 2 : 1), &ws);
+        if (e != hipSuccess) return st_of(e);
+        unsigned short *P = (unsigned short *)ws;
+        unsigned short *Pl = x3 ? (unsigned short *)((char *)ws + bytes) : nullptr;
+        e = timed(h, CURFP_KERNEL_PACK, 0.0, (double)n * K * 4.0 + (double)n * Kp * (x3 ? 4.0 : 2.0), [&] {
+            return launch_to_bf16_rows((const float *)A, av, n, K, Kp, Kp, P, Pl, h->stream);
+        });
+        if (e != hipSuccess) return st_of(e);
+        const int64_t r0[3] = {0, g.n1, g.n1}, c0[3] = {0, 0, g.n1};
+        for (int q = 0; q < 3; ++q) {
+            const Blk &bk = blk[q];
+            if (bk.M <= 0 || bk.N <= 0) continue;
+            GemmArgs ga = gemm_args();
+            ga.a = P; ga.a_lo = Pl; ga.va = RfpView{r0[q] * Kp, Kp, 1};
+            ga.b = P; ga.b_lo = Pl; ga.vb = RfpView{c0[q] * Kp, Kp, 1};
+            ga.bf16_rows = true;
+            ga.c = C; ga.vc = bk.c;
+            ga.M = bk.M; ga.N = bk.N; ga.K = Kp;
+            ga.alpha = (double)a; ga.beta = (double)b;
+            ga.lower = bk.lower;
+            e = run_gemm<T>(h, ga, m);
+            if (e != hipSuccess) return st_of(e);
+        }
+        return CURFP_STATUS_SUCCESS;
+    }
     for (const Blk &bk : blk) {
         if (bk.M <= 0 || bk.N <= 0) continue;
         GemmArgs ga = gemm_args();

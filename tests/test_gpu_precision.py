@@ -76,9 +76,11 @@ def test_ssfrk_bf16_input_large_tiles(curfp, n, k, ld, t, u, beta):
     assert np.linalg.norm(Cd - S) <= 4 * math.sqrt(k) * u32 * np.linalg.norm(S)
 
 
+@pytest.mark.parametrize("n,k", [(777, 300), (3000, 300), (2500, 77)])
 @pytest.mark.parametrize("t,u", VARIANTS)
-def test_ssfrk_math_modes(curfp, t, u):
-    n, k = 777, 300
+def test_ssfrk_math_modes(curfp, t, u, n, k):
+    """FP32 A in the three math modes; from n = 2048 (k >= 32) the BF16 modes
+    convert A once to BF16 row blocks (k = 77 padded to 80)."""
     A, _ = problem(n, k, 1, seed=11)
     Ad = torch.from_numpy(A).cuda()
     S = A.astype(np.float64) @ A.astype(np.float64).T
