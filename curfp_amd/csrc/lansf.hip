@@ -11,15 +11,17 @@
  * FP64 sums), then a pass over the n diagonal entries: ||A||_F^2 =
  * 2 sum(arf^2) - sum(diag^2).
  *
- * 1-norm: the stored triangle L (lower, as the three views) gives column
- * sums of the full matrix as  s_j = sum_{i >= j} |L_ij| + sum_{k < j} |L_jk|.
- * One workgroup per 128 x 128 tile (bi, bj), bi >= bj, of the 128-row blocks
- * of v11 then v22 (the block structure of the substitution sweep) reads the
- * tile once and writes its column sums to S[bi][cols of bj] and its strict
- * row sums to S[bj][rows of bi] (a diagonal tile writes both into
- * S[bi][block bi]).  Every (slot, index) pair of S is written by exactly one
- * tile, so S needs no clearing and no atomics; a second kernel sums the
- * slots of each index in FP64 and takes the maximum.
+ * 1-norm: the stored triangle L (lower, as the three views) gives the
+ * absolute row sums of the full matrix as  s_i = sum_{j <= i} |L_ij| +
+ * sum_{k > i} |L_ki|.  One workgroup per 128 x 128 tile (bi, bj), bi >= bj,
+ * of the 128-row blocks of v11 then v22 (the block structure of the
+ * substitution sweep) reads the tile once and writes its row sums to
+ * S[bj][rows of bi] and its strict column sums to S[bi][cols of bj] (a
+ * diagonal tile writes both into S[bi][block bi]) — the symmetric
+ * matrix-vector kernel below with |L| and x = 1.  Every (slot, index) pair
+ * of S is written by exactly one tile, so S needs no clearing and no
+ * atomics; a second kernel sums the slots of each index in FP64 and takes
+ * the maximum.
  */
 #include "rfp_internal.h"
 #include <algorithm>
@@ -27,7 +29,6 @@
 namespace {
 
 constexpr int NT = 128;       /* tile of the 1-norm */
-constexpr int TPP = NT + 1;   /* LDS pitch */
 
 struct LansfGeom {
     RfpView v11, v21, v22;
@@ -139,81 +140,6 @@ __global__ __launch_bounds__(256) void diag_sumsq_kernel(const T *a, LansfGeom g
     if (threadIdx.x == 0) atomicAdd(reinterpret_cast<double *>(acc + 2), s);
 }
 
-/* column / strict row sums of one tile of |L| into S (slot * n + index) */
-template <typename T>
-__global__ __launch_bounds__(256) void colsum_tiles_kernel(const T *a, LansfGeom g, T *S)
-{
-    __shared__ T tile[NT * TPP];
-    __shared__ T csum[NT];
-    const int tid = threadIdx.x;
-    int64_t bi, bj;
-    tri_tile_coords((int64_t)blockIdx.x, g.nb, bi, bj);   /* bi >= bj */
-    int64_t gr, lr, gc, lc;
-    int mr, mc;
-    lansf_block(g, (int)bi, gr, lr, mr);
-    lansf_block(g, (int)bj, gc, lc, mc);
-    const RfpView v = bi < g.s1 ? g.v11.sub(lr, lc) : (bj < g.s1 ? g.v21.sub(lr, lc) : g.v22.sub(lr, lc));
-    const bool diag = bi == bj;
-    const bool rows_contig = v.rs == 1;
-    typedef typename Vec16<T>::type V;
-    constexpr int VN = Vec16<T>::N;
-    constexpr int PER = NT * NT / VN / 256;    /* 16-byte pieces per thread */
-    constexpr int RUN = NT / VN;               /* pieces along the contiguous direction */
-    if (!diag && mr == NT && mc == NT) {
-        /* interior tile: every load issued before the LDS stores; piece q
-         * covers VN consecutive entries along the contiguous direction */
-        V x[PER];
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int q = tid + 256 * e;
-            const int a0 = (q % RUN) * VN, b0 = q / RUN;
-            const T *p = a + (rows_contig ? v.at(a0, b0) : v.at(b0, a0));
-            typedef T Vu __attribute__((ext_vector_type(VN), aligned(sizeof(T))));
-            const Vu u = *(const Vu *)p;
-#pragma unroll
-            for (int k = 0; k < VN; ++k) x[e][k] = fabs(u[k]);
-        }
-#pragma unroll
-        for (int e = 0; e < PER; ++e) {
-            const int q = tid + 256 * e;
-            const int a0 = (q % RUN) * VN, b0 = q / RUN;
-#pragma unroll
-            for (int k = 0; k < VN; ++k) {
-                if (rows_contig) tile[(a0 + k) * TPP + b0] = x[e][k];
-                else             tile[b0 * TPP + a0 + k] = x[e][k];
-            }
-        }
-    } else {
-        /* edge or diagonal tile; entries above a diagonal tile's diagonal
-         * belong to other blocks of the packed array and are not read */
-#pragma unroll 8
-        for (int e = 0; e < NT * NT / 256; ++e) {
-            const int idx = tid + 256 * e;
-            const int r = rows_contig ? idx % NT : idx / NT;
-            const int c = rows_contig ? idx / NT : idx % NT;
-            T x = 0;
-            if (r < mr && c < mc && (!diag || r >= c)) x = fabs(a[v.at(r, c)]);
-            tile[r * TPP + c] = x;
-        }
-    }
-    __syncthreads();
-    if (tid < NT) {
-        const int c = tid;
-        T s = 0;
-        for (int r = 0; r < NT; ++r) s += tile[r * TPP + c];
-        if (diag) csum[c] = s;
-        else if (c < mc) S[bi * g.n + gc + c] = s;
-    }
-    if (diag) __syncthreads();
-    if (tid >= NT) {
-        const int r = tid - NT;
-        T s = 0;
-        const int cend = diag ? r : NT;   /* strict: the diagonal is in the column sum */
-        for (int c = 0; c < cend; ++c) s += tile[r * TPP + c];
-        if (r < mr) S[bj * g.n + gr + r] = diag ? s + csum[r] : s;
-    }
-}
-
 /* acc[3] (bits) = max_j sum_slot S[slot][j] */
 template <typename T>
 __global__ __launch_bounds__(256) void colsum_max_kernel(const T *S, LansfGeom g, unsigned long long *acc)
@@ -256,7 +182,7 @@ __global__ __launch_bounds__(256) void colsum_max_kernel(const T *S, LansfGeom g
  * pieces for the VN lines of the contiguous direction (reduced over the
  * 256 / RUN thread groups through LDS).
  */
-template <typename T>
+template <typename T, bool ABS1>
 __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g, const T *x, RfpView xv, T *S)
 {
     typedef typename Vec16<T>::type V;
@@ -289,7 +215,7 @@ __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g
             const int b0 = tid / RUN + GROUPS * e;
             const Vu w = *(const Vu *)(a + (rc ? v.at(a0, b0) : v.at(b0, a0)));
 #pragma unroll
-            for (int k = 0; k < VN; ++k) u_all[e][k] = w[k];
+            for (int k = 0; k < VN; ++k) u_all[e][k] = ABS1 ? fabs(w[k]) : w[k];
         }
     } else {
 #pragma unroll
@@ -298,12 +224,14 @@ __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g
 #pragma unroll
             for (int k = 0; k < VN; ++k) {
                 const int r = rc ? a0 + k : b0, c = rc ? b0 : a0 + k;
-                u_all[e][k] = (r < mr && c < mc && (!diag || r >= c)) ? a[v.at(r, c)] : T(0);
+                const T w = (r < mr && c < mc && (!diag || r >= c)) ? a[v.at(r, c)] : T(0);
+                u_all[e][k] = ABS1 ? fabs(w) : w;
             }
         }
     }
-    if (tid < NT) xr[tid] = tid < mr ? x[xv.off + (gr + tid) * xv.rs] : T(0);
-    else          xc[tid - NT] = tid - NT < mc ? x[xv.off + (gc + tid - NT) * xv.rs] : T(0);
+    /* ABS1: the 1-norm's absolute row sums, x = 1 (not read) */
+    if (tid < NT) xr[tid] = tid < mr ? (ABS1 ? T(1) : x[xv.off + (gr + tid) * xv.rs]) : T(0);
+    else          xc[tid - NT] = tid - NT < mc ? (ABS1 ? T(1) : x[xv.off + (gc + tid - NT) * xv.rs]) : T(0);
     __syncthreads();
 #pragma unroll
     for (int e = 0; e < PER; ++e) {
@@ -390,7 +318,7 @@ hipError_t symv(const RfpGeom &rg, const T *arf, const T *x, RfpView xv, T alpha
     int nslots = 0;
     if (alpha != T(0)) {
         const int64_t ntiles = (int64_t)g.nb * (g.nb + 1) / 2;
-        hipLaunchKernelGGL((symv_tiles_kernel<T>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, x, xv, S);
+        hipLaunchKernelGGL((symv_tiles_kernel<T, false>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, x, xv, S);
         nslots = g.nb;
     }
     const unsigned blocks = (unsigned)std::min<int64_t>((g.n + 255) / 256, 2048);
@@ -423,7 +351,8 @@ hipError_t lansf(const RfpGeom &rg, int norm, const T *arf, void *ws, hipStream_
     if (norm == 1) {
         T *S = (T *)((char *)ws + 256);
         const int64_t ntiles = (int64_t)g.nb * (g.nb + 1) / 2;
-        hipLaunchKernelGGL((colsum_tiles_kernel<T>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, S);
+        hipLaunchKernelGGL((symv_tiles_kernel<T, true>), dim3((unsigned)ntiles), dim3(256), 0, s, arf, g, (const T *)nullptr,
+                           RfpView{0, 1, 1}, S);
         const unsigned blocks = (unsigned)std::min<int64_t>((g.n + 255) / 256, 2048);
         hipLaunchKernelGGL((colsum_max_kernel<T>), dim3(blocks), dim3(256), 0, s, S, g, acc);
     } else {
