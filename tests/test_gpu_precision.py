@@ -234,3 +234,30 @@ def test_refinement_bf16_solves(curfp, t, u, nrhs):
     r = O.dense_relative_residual(S, X.cpu().numpy(), B.astype(np.float64))
     assert info["rel_residual"] <= 1e-6 and r <= 1.5e-6, (info, r)
     assert info["solves"] <= 4, info
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16"])
+@pytest.mark.parametrize("t,u", [("N", "L"), ("T", "U")])
+def test_refinement_general_rfp(curfp, mode, t, u):
+    """refine_solve_rfp: a general SPD RFP matrix (no generator) in FP64, its
+    FP32 factor in the given math mode, the residual from dsfmm; the relative
+    residual is checked independently on the host in float64."""
+    from curfp_amd.refine import refine_solve_rfp
+
+    n, k, nrhs = 1500, 700, 3
+    rng = np.random.default_rng(5)
+    G = rng.standard_normal((n, k)) / math.sqrt(k)
+    S = G @ G.T + 0.05 * np.eye(n)
+    C64 = torch.from_numpy(O.trttf(S, t, u)).cuda()
+    F = C64.to(torch.float32)
+    with curfp.math_mode(mode):
+        assert curfp.spftrf(F, transr=t, uplo=u) == 0
+    B = rng.standard_normal((n, nrhs))
+    for tol in (1e-6, 1e-12):
+        X, info = refine_solve_rfp(F, C64, torch.from_numpy(B).cuda(), tol=tol, transr=t, uplo=u)
+        Xh = X.cpu().numpy()
+        res = np.linalg.norm(B - S @ Xh) / (np.linalg.norm(S) * np.linalg.norm(Xh))
+        assert res <= tol, (tol, res, info)
+        assert abs(res - info["rel_residual"]) <= 0.01 * res + 1e-17
+    x, info = refine_solve_rfp(F, C64, torch.from_numpy(B[:, 0]).cuda(), tol=1e-10, transr=t, uplo=u)
+    assert x.shape == (n,) and info["rel_residual"] <= 1e-10
