@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void absmax_sumsq_kernel(const T *a, int64_t n
     }
     const V *av = reinterpret_cast<const V *>(a + head);
     for (int64_t i = tid; i < nv; i += stride) {
-        const V v = av[i];
+        const V v = __builtin_nontemporal_load(av + i);   /* read once */
 #pragma unroll
         for (int e = 0; e < VN; ++e) {
             const T x = fabs(v[e]);
@@ -213,7 +213,8 @@ __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g
 #pragma unroll
         for (int e = 0; e < PER; ++e) {
             const int b0 = tid / RUN + GROUPS * e;
-            const Vu w = *(const Vu *)(a + (rc ? v.at(a0, b0) : v.at(b0, a0)));
+            /* read once: non-temporal */
+            const Vu w = __builtin_nontemporal_load((const Vu *)(a + (rc ? v.at(a0, b0) : v.at(b0, a0))));
 #pragma unroll
             for (int k = 0; k < VN; ++k) u_all[e][k] = ABS1 ? fabs(w[k]) : w[k];
         }
