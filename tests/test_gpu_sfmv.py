@@ -119,3 +119,21 @@ def test_dsfmv_matches_oracle(curfp, n, t, u):
                 uplo=u)
     ref = ref_mv(arf, n, x, y0, 0.7, -1.2, t, u)
     assert np.abs(yd.cpu().numpy() - ref).sum() <= bound(n, S, x, y0, 0.7, -1.2, 2.0 ** -53)
+
+
+@pytest.mark.parametrize("n,t,u", [(65537, "T", "U"), (65536, "N", "L")])
+def test_ssfmv_64bit_sizes_closed_form(curfp, n, t, u):
+    """n(n+1)/2 above 2^31 entries: A = c J + d I, x_i = 1 + i % 3, so
+    y_i = c sum(x) + d x_i (every partial sum exact in FP32)."""
+    c, d = 0.5, 2.0
+    nt = n * (n + 1) // 2
+    assert nt > 2 ** 31
+    C = torch.full((nt,), c, device="cuda")
+    curfp.add_to_diagonal(C, d, transr=t, uplo=u)
+    xi = 1.0 + (np.arange(n) % 3)
+    x = torch.from_numpy(xi.astype(np.float32)).cuda()
+    y = curfp.ssfmv(C, x, transr=t, uplo=u).cpu().numpy().astype(np.float64)
+    ref = c * xi.sum() + d * xi
+    assert np.abs(y - ref).max() <= 1e-6 * np.abs(ref).max(), np.abs(y - ref).max()
+    del C
+    torch.cuda.empty_cache()
