@@ -175,13 +175,31 @@ __global__ __launch_bounds__(256) void colsum_max_kernel(const T *S, LansfGeom g
  * bj, bi >= bj) contributes L x_bj to the rows of bi (slot bj) and, off the
  * diagonal, L^T x_bi to the columns of bj (slot bi); a diagonal tile writes
  * L x + strict(L)^T x into slot bi.  Every (slot, index) pair of S is
- * written by exactly one tile.  A thread's 16-byte piece runs along the
+ * written by exactly one tile.  Tiles are taken strip by strip (all tiles of
+ * column block 0 top to bottom, then column block 1, ...): consecutive
+ * workgroups then read consecutive 128-row pieces of the same columns, which
+ * measured 1.3x faster than the SYRK launches' banded order (DRAM page
+ * locality; every byte is read once, so L2 reuse plays no part).  A thread's 16-byte piece runs along the
  * contiguous direction: its products summed within the piece give one
  * partial of the other direction (reduced over the RUN lanes that share the
  * piece's line), and the products themselves accumulate over the thread's
  * pieces for the VN lines of the contiguous direction (reduced over the
  * 256 / RUN thread groups through LDS).
  */
+/* Tile t of the lower triangle in column-strip order: strip tj = 0, 1, ..,
+ * and down each strip ti = tj, tj + 1, .. (consecutive workgroups read
+ * consecutive 128-row pieces of the same columns). */
+__device__ __forceinline__ void strip_tile_coords(int64_t t, int64_t T, int64_t &ti, int64_t &tj)
+{
+    const int64_t u = T * (T + 1) / 2 - 1 - t;     /* row-major index in the reversed triangle */
+    int64_t r = (int64_t)((sqrt(8.0 * (double)u + 1.0) - 1.0) * 0.5);
+    while (r > 0 && r * (r + 1) / 2 > u) --r;
+    while ((r + 1) * (r + 2) / 2 <= u) ++r;
+    const int64_t c = u - r * (r + 1) / 2;
+    tj = T - 1 - r;
+    ti = T - 1 - c;
+}
+
 template <typename T, bool ABS1>
 __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g, const T *x, RfpView xv, T *S)
 {
@@ -193,7 +211,7 @@ __global__ __launch_bounds__(256) void symv_tiles_kernel(const T *a, LansfGeom g
     __shared__ T xr[NT], xc[NT], line[NT], red[GROUPS][NT];
     const int tid = threadIdx.x;
     int64_t bi, bj;
-    tri_tile_coords((int64_t)blockIdx.x, g.nb, bi, bj);
+    strip_tile_coords((int64_t)blockIdx.x, g.nb, bi, bj);
     int64_t gr, lr, gc, lc;
     int mr, mc;
     lansf_block(g, (int)bi, gr, lr, mr);
