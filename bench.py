@@ -16,7 +16,7 @@ Two pipelines are timed:
 divided by the wall time of one step) of the fastest pipeline that reaches
 the residual target; both pipelines are reported.
 
-Prints ONE JSON line.  --gpus N > 1 is refused: this build targets one GPU.
+Prints ONE JSON line.  --gpus N > 1 is refused: curfp_amd runs on one GPU.
 """
 from __future__ import annotations
 
@@ -448,7 +448,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
+        print(json.dumps({"error": "curfp_amd runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}), flush=True)
         return 2
 

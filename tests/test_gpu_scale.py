@@ -1,22 +1,23 @@
-"""The BASELINE workloads at full size on one MI355X (parity at scale).
+"""The reference's benchmark workloads at full size on one MI355X (parity at scale).
 
-  config 2 — plain method at scale: all 8 storage variants at n = 65536 and
+  plain method at scale: all 8 storage variants at n = 65536 and
              65535 (k = 1024, lambda = 1, 64 rhs, FP32), the FP64 d* chain at
              n = 32768, and a bit-exact strttf/stfttr round trip at n = 32768;
-  config 3 — precision: n = 90000, k = 4096, BF16 A (and its FP32 copy),
+  precision: n = 90000, k = 4096, BF16 A (and its FP32 copy),
              lambda = 1e-2, 128 rhs: BF16, split-BF16 and FP32 factors, each
              refined in FP64 to a relative residual <= 1e-6; forward error
              against an FP64 dpftrf/dpftrs solution at n = 16384;
-  config 4 — differentiation cost: forward + backward within 3x forward at
+  differentiation cost: forward + backward within 3x forward at
              n = 16384, k = 512, 16 rhs;
-  config 5 — combined: n = 100000, k = 2048, 128 rhs, BF16 A, lambda = 1e-2,
+  combined: n = 100000, k = 2048, 128 rhs, BF16 A, lambda = 1e-2,
              split-BF16 factor + FP64 refinement, forward + backward with the
              factor recomputed instead of stored; gradient error against FP64
              on a 4096-row problem of the same distribution.
 
 Residuals are ||B - C X||_F / (||C||_F ||X||_F) formed in FP64 on the GPU
-from the generator A.  Each test prints its figures (pytest -s) and writes
-them to gpurun_out/scale_results.json when that directory exists.
+from the generator A.  Each test prints its figures (pytest -s) and, when the
+environment variable CURFP_SCALE_OUT names a directory, writes them to
+scale_results.json there.
 """
 import json
 import math
@@ -37,8 +38,8 @@ def curfp(cuda):
     import curfp_amd
 
     yield curfp_amd
-    out = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
-    if os.path.isdir(out) and RESULTS:
+    out = os.environ.get("CURFP_SCALE_OUT", "")
+    if out and os.path.isdir(out) and RESULTS:
         with open(os.path.join(out, "scale_results.json"), "w") as f:
             json.dump(RESULTS, f, indent=1)
 
@@ -78,7 +79,7 @@ def timed(fn):
 
 
 @pytest.mark.parametrize("n", [65536, 65535])
-def test_config2_all_variants(curfp, n):
+def test_plain_all_variants_n65536(curfp, n):
     k, nrhs, lam = 1024, 64, 1.0
     A, B = gen(n, k, nrhs, seed=1)
     nc = norm_C(A, lam)
@@ -100,7 +101,7 @@ def test_config2_all_variants(curfp, n):
             assert info == 0 and r <= 1e-6
 
 
-def test_config2_fp64_and_pack(curfp):
+def test_plain_fp64_and_pack_n32768(curfp):
     n, k, nrhs, lam = 32768, 1024, 64, 1.0
     A32, B32 = gen(n, k, nrhs, seed=1)
     A, B = A32.double(), B32.double()
@@ -130,7 +131,7 @@ def test_config2_fp64_and_pack(curfp):
     record("c2_pack_roundtrip_n32768", bit_exact=True)
 
 
-def test_config3_precision(curfp):
+def test_precision_n90000_k4096(curfp):
     n, k, nrhs, lam = 90000, 4096, 128, 1e-2
     A, B = gen(n, k, nrhs, seed=2, bf16=True)
     nc = norm_C(A, lam)
@@ -195,7 +196,7 @@ def test_config3_precision(curfp):
             assert err <= 5e-2 and err <= err_plain * (1 + 1e-9)
 
 
-def test_config4_backward_cost(curfp):
+def test_backward_cost_n16384(curfp):
     from curfp_amd.autograd import krr_solve
 
     n, k, nrhs, lam = 16384, 512, 16, 0.1
@@ -213,7 +214,7 @@ def test_config4_backward_cost(curfp):
     assert (t_fwd + t_bwd) <= 3.0 * t_fwd
 
 
-def test_config5_combined(curfp):
+def test_combined_n100000_k2048(curfp):
     from curfp_amd.autograd import krr_solve
 
     n, k, nrhs, lam = 100000, 2048, 128, 1e-2
