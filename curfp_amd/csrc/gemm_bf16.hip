@@ -482,6 +482,35 @@ __global__ __launch_bounds__(256) void to_bf16_rows_kernel(const float *A, RfpVi
     }
 }
 
+/* The same for an operand whose rows are contiguous (cs != 1, rs == 1): a
+ * 64 x 64 tile goes through LDS, read along rows and written along k. */
+__global__ __launch_bounds__(256) void to_bf16_rows_t_kernel(const float *A, RfpView av, int64_t rows, int64_t K,
+                                                             int64_t Kp, int64_t ldo, unsigned short *hi,
+                                                             unsigned short *lo)
+{
+    __shared__ float tile[64][65];
+    const int tid = threadIdx.x;
+    const int64_t p0 = (int64_t)blockIdx.x * 64, i0 = (int64_t)blockIdx.y * 64;
+#pragma unroll 4
+    for (int e = 0; e < 16; ++e) {
+        const int ii = tid % 64, pp = tid / 64 + 4 * e;
+        const int64_t i = i0 + ii, p = p0 + pp;
+        tile[pp][ii] = (i < rows && p < K) ? A[av.off + i * av.rs + p * av.cs] : 0.f;
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int e = 0; e < 16; ++e) {
+        const int pp = tid % 64, ii = tid / 64 + 4 * e;
+        const int64_t i = i0 + ii, p = p0 + pp;
+        if (i < rows && p < Kp) {
+            const float x = tile[pp][ii];
+            const unsigned short h = bf16_bits(x);
+            hi[i * ldo + p] = h;
+            if (lo) lo[i * ldo + p] = bf16_bits(x - to_f(h));
+        }
+    }
+}
+
 }  // namespace
 
 hipError_t launch_to_bf16_rows(const float *A, RfpView av, int64_t rows, int64_t K, int64_t Kp, int64_t ldo,
@@ -489,6 +518,11 @@ hipError_t launch_to_bf16_rows(const float *A, RfpView av, int64_t rows, int64_t
 {
     if (rows <= 0 || Kp <= 0) return hipSuccess;
     if (K > Kp || Kp > ldo) return hipErrorInvalidValue;
+    if (av.cs != 1 && av.rs == 1 && (rows + 63) / 64 <= 65535) {
+        const dim3 grid((unsigned)((Kp + 63) / 64), (unsigned)((rows + 63) / 64));
+        hipLaunchKernelGGL(to_bf16_rows_t_kernel, grid, dim3(256), 0, s, A, av, rows, K, Kp, ldo, hi, lo);
+        return hipGetLastError();
+    }
     const int64_t total = rows * Kp;
     const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
     hipLaunchKernelGGL(to_bf16_rows_kernel, dim3(blocks), dim3(256), 0, s, A, av, rows, K, Kp, ldo, hi, lo);
