@@ -158,11 +158,10 @@ __device__ __forceinline__ void mma_slice(f32x16 (&acc)[Tile<BT>::F][Tile<BT>::F
                 f[nxt].a[i] = *(const bf16x8 *)(arow + i * 32 * ROWB + o);
                 f[nxt].b[i] = *(const bf16x8 *)(brow + i * 32 * ROWB + o);
             }
-            if (t == 0) {
-                da.issue(Na, rsa, lda2, k_next, K, 2 * i, wave);
-                da.issue(Na, rsa, lda2, k_next, K, 2 * i + 1, wave);
-                db.issue(Nb, rsb, ldb2, k_next, K, 2 * i, wave);
-                db.issue(Nb, rsb, ldb2, k_next, K, 2 * i + 1, wave);
+            if (t < 2) {
+                /* the next slice's 2 F groups per operand, spread over steps 0 and 1 */
+                da.issue(Na, rsa, lda2, k_next, K, t * F + i, wave);
+                db.issue(Nb, rsb, ldb2, k_next, K, t * F + i, wave);
             }
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
