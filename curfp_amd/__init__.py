@@ -39,6 +39,11 @@ try:
         MATH_FP32,
         MATH_BF16,
         MATH_BF16X3,
+        NORM_MAX,
+        NORM_ONE,
+        NORM_FRO,
+        SIDE_LEFT,
+        SIDE_RIGHT,
         KERNEL_CLASSES,
         ssfrk as _ssfrk_C,
         ssfrk_bf16 as _ssfrk_bf16_C,
@@ -82,11 +87,6 @@ except ImportError as exc:  # pragma: no cover - exercised only without a build
         "(hipcc --offload-arch=gfx950) first"
     ) from exc
 
-# Enum values of the reference module (curfp/_ext.cpp:869-877): the norm types
-# of slansf / dlansf, and the sides of ssfmm (outside the path built here).
-NORM_MAX, NORM_ONE, NORM_FRO = 0, 1, 2
-SIDE_LEFT, SIDE_RIGHT = 0, 1
-
 # norm strings accepted by slansf / dlansf (curfp/__init__.py:114-123)
 _NORM_MAP = {"M": NORM_MAX, "MAX": NORM_MAX, "1": NORM_ONE, "O": NORM_ONE, "I": NORM_ONE,
              "F": NORM_FRO, "E": NORM_FRO, "FRO": NORM_FRO}
@@ -120,6 +120,13 @@ def set_stream(stream) -> None:
 
 
 _MATH_MAP = {"fp32": MATH_FP32, "bf16": MATH_BF16, "bf16x3": MATH_BF16X3}
+
+
+def synchronize() -> None:
+    """Wait for the current device's curfp stream; raises RuntimeError
+    ("curfp error: status=4") if an earlier asynchronous call failed on the
+    device (a substitution sweep that gave up waiting and wrote NaN)."""
+    _get_handle().synchronize()
 
 
 def set_math_mode(mode: str) -> None:

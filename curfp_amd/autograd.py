@@ -48,7 +48,10 @@ def _factor(A: torch.Tensor, lam: float, alpha: float, transr: str, uplo: str, m
 
 
 def _solve(C: torch.Tensor, B: torch.Tensor, A: torch.Tensor, lam: float, alpha: float, transr: str, uplo: str,
-           refine_tol, max_iter: int) -> torch.Tensor:
+           refine_tol, max_iter: int, strict: bool = True):
+    """X = C^{-1} B with the factor C; with refine_tol, FP64 refinement
+    against the generator.  Returns X, or None when the refinement misses
+    refine_tol and strict is False."""
     if refine_tol is None:
         X = B.to(C.dtype).contiguous().clone()
         (_c.dpftrs if C.dtype == torch.float64 else _c.spftrs)(C, X, transr=transr, uplo=uplo)
@@ -57,8 +60,31 @@ def _solve(C: torch.Tensor, B: torch.Tensor, A: torch.Tensor, lam: float, alpha:
 
     X, info = refine_solve(C, A, lam, B, alpha=alpha, tol=refine_tol, max_iter=max_iter, transr=transr, uplo=uplo)
     if not info["rel_residual"] <= refine_tol:
+        if not strict:
+            return None
         raise RuntimeError(f"iterative refinement did not reach {refine_tol:g}: {info}")
     return X
+
+
+def _factor_solve(A, lam, alpha, transr, uplo, math, Y, refine_tol, max_iter):
+    """Factor and solve with the precision policy of the BF16 mode: a BF16
+    factor that is not positive definite, or whose refined solve (classical
+    steps, then flexible GMRES) misses refine_tol, is replaced by the
+    split-BF16 (FP32-accurate) factor.  Returns (factor, X, math used)."""
+    try:
+        C = _factor(A, lam, alpha, transr, uplo, math)
+    except torch.linalg.LinAlgError:
+        if math != "bf16":
+            raise
+        math = "bf16x3"
+        C = _factor(A, lam, alpha, transr, uplo, math)
+    X = _solve(C, Y, A, lam, alpha, transr, uplo, refine_tol, max_iter, strict=math != "bf16")
+    if X is None:
+        del C
+        math = "bf16x3"
+        C = _factor(A, lam, alpha, transr, uplo, math)
+        X = _solve(C, Y, A, lam, alpha, transr, uplo, refine_tol, max_iter)
+    return C, X, math
 
 
 class KRRSolve(torch.autograd.Function):
@@ -68,8 +94,7 @@ class KRRSolve(torch.autograd.Function):
     def forward(ctx, A, lam, Y, alpha, transr, uplo, recompute, math, refine_tol, max_iter):
         A = A.contiguous()
         lam_f = float(lam)
-        C = _factor(A, lam_f, alpha, transr, uplo, math)
-        X = _solve(C, Y, A, lam_f, alpha, transr, uplo, refine_tol, max_iter)
+        C, X, math = _factor_solve(A, lam_f, alpha, transr, uplo, math, Y, refine_tol, max_iter)
         ctx.alpha, ctx.lam, ctx.transr, ctx.uplo = alpha, lam_f, transr, uplo
         ctx.math, ctx.refine_tol, ctx.max_iter = math, refine_tol, max_iter
         ctx.lam_is_tensor = torch.is_tensor(lam)
@@ -117,7 +142,9 @@ def krr_solve(A: torch.Tensor, lam, Y: torch.Tensor, *, alpha: float = 1.0, tran
     A: (n, k) float32 or float64 device tensor (row-major, the ssfrk default
     trans='T').  lam: float or 0-d tensor.  Y: (n, nrhs) or (n,).
     math: arithmetic of ssfrk and the spftrf updates for a float32 A
-    ('fp32', 'bf16': A rounded to BF16, or 'bf16x3').  refine_tol: when set,
+    ('fp32', 'bf16': A rounded to BF16, or 'bf16x3'); a 'bf16' factor that
+    fails (not positive definite, or refinement short of refine_tol) is
+    redone in 'bf16x3', which the backward then reuses.  refine_tol: when set,
     both the forward solve and the adjoint solve use FP64 iterative
     refinement against the FP32 generator down to that relative residual, and
     X (and the gradients w.r.t. Y) are float64.

@@ -30,6 +30,15 @@ __global__ void acc_f32_f64_kernel(int64_t n, int64_t nrhs, const float *D, int6
     *x = accumulate ? *x + d : d;
 }
 
+/* Y = X, both n x nrhs column-major FP64 */
+__global__ void copy_f64_kernel(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *Y, int64_t ldy)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t i = idx % n, c = idx / n;
+    Y[i + c * ldy] = X[i + c * ldx];
+}
+
 /* out[slot] += sum of squares of an n x nrhs column-major FP64 matrix */
 __global__ void sumsq_kernel(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out)
 {
@@ -111,9 +120,82 @@ __global__ void gram_f32_kernel(int64_t k, const float *G, double *out)
     }
 }
 
+/* ---- flexible GMRES on n x nrhs blocks (one Krylov space per column) ----
+ * Vector j of a basis is the column-major n x nrhs FP64 block at
+ * base + j * n * nrhs; every column c of every kernel below is independent. */
+
+/* Partial dot products of one GM_CHUNK-row piece of column c (blockIdx.y)
+ * with each of nv basis vectors: part[(chunk * nv + i) * nrhs + c] =
+ * sum_r V_i(r, c) w(r, c).  w's piece is staged in LDS once. */
+constexpr int GM_CHUNK = 2048;
+__global__ __launch_bounds__(256) void gm_dots_kernel(int64_t n, int64_t nrhs, const double *V, int nv,
+                                                      const double *w, double *part)
+{
+    __shared__ double ws[GM_CHUNK];
+    __shared__ double red[4];
+    const int64_t c = blockIdx.y, r0 = (int64_t)blockIdx.x * GM_CHUNK;
+    const int rows = (int)min<int64_t>(GM_CHUNK, n - r0);
+    const int tid = threadIdx.x;
+    const double *wc = w + c * n + r0;
+    for (int r = tid; r < rows; r += 256) ws[r] = wc[r];
+    __syncthreads();
+    for (int i = 0; i < nv; ++i) {
+        const double *vc = V + (int64_t)i * n * nrhs + c * n + r0;
+        double sum = 0.0;
+        for (int r = tid; r < rows; r += 256) sum += vc[r] * ws[r];
+        for (int o = 32; o > 0; o >>= 1) sum += __shfl_xor(sum, o);
+        if ((tid & 63) == 0) red[tid >> 6] = sum;
+        __syncthreads();
+        if (tid == 0) part[((int64_t)blockIdx.x * nv + i) * nrhs + c] = (red[0] + red[1]) + (red[2] + red[3]);
+        __syncthreads();
+    }
+}
+
+/* out[t] = sum over chunks of part[chunk * cnt + t], in chunk order */
+__global__ void gm_reduce_kernel(int nchunks, int64_t cnt, const double *part, double *out)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= cnt) return;
+    double s = 0.0;
+    for (int ch = 0; ch < nchunks; ++ch) s += part[(int64_t)ch * cnt + t];
+    out[t] = s;
+}
+
+/* w(r, c) -= sum_{i < nv} h[i * nrhs + c] V_i(r, c) */
+__global__ void gm_update_kernel(int64_t n, int64_t nrhs, const double *V, int nv, const double *h, double *w)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t c = idx / n;
+    double s = 0.0;
+    for (int i = 0; i < nv; ++i) s += h[(int64_t)i * nrhs + c] * V[(int64_t)i * n * nrhs + idx];
+    w[idx] -= s;
+}
+
+/* v(r, c) = w(r, c) / sqrt(nrm2[c]) (zero for a zero column) */
+__global__ void gm_normalize_kernel(int64_t n, int64_t nrhs, const double *w, const double *nrm2, double *v)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const double q = nrm2[idx / n];
+    v[idx] = q > 0.0 ? w[idx] / sqrt(q) : 0.0;
+}
+
+/* X(r, c) += sum_{j < nz} y[j * nrhs + c] Z_j(r, c) */
+__global__ void gm_combine_kernel(int64_t n, int64_t nrhs, const double *Z, int nz, const double *y, double *X,
+                                  int64_t ldx)
+{
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n * nrhs) return;
+    const int64_t r = idx % n, c = idx / n;
+    double s = 0.0;
+    for (int j = 0; j < nz; ++j) s += y[(int64_t)j * nrhs + c] * Z[(int64_t)j * n * nrhs + idx];
+    X[r + c * ldx] += s;
+}
+
 __global__ void zero_f64_kernel(double *p, int n)
 {
-    if ((int)threadIdx.x < n) p[threadIdx.x] = 0.0;
+    for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0.0;
 }
 
 inline unsigned blocks_for(int64_t total, int64_t cap = 1 << 20)
@@ -142,6 +224,14 @@ hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t l
     return hipGetLastError();
 }
 
+hipError_t launch_copy_f64(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(copy_f64_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, X, ldx, Y,
+                       ldy);
+    return hipGetLastError();
+}
+
 hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(sumsq_kernel, dim3(blocks_for(n * nrhs, 2048)), dim3(256), 0, s, n, nrhs, X, ldx, out);
@@ -163,5 +253,40 @@ hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_
 hipError_t launch_zero_f64(double *p, int n, hipStream_t s)
 {
     hipLaunchKernelGGL(zero_f64_kernel, dim3(1), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
+int64_t gm_chunks(int64_t n) { return (n + GM_CHUNK - 1) / GM_CHUNK; }
+
+hipError_t launch_gm_dots(int64_t n, int64_t nrhs, const double *V, int nv, const double *w, double *part,
+                          double *out, hipStream_t s)
+{
+    const int64_t nch = gm_chunks(n);
+    if (nv <= 0 || nrhs > 65535 || nch > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(gm_dots_kernel, dim3((unsigned)nch, (unsigned)nrhs), dim3(256), 0, s, n, nrhs, V, nv, w, part);
+    const int64_t cnt = (int64_t)nv * nrhs;
+    hipLaunchKernelGGL(gm_reduce_kernel, dim3(blocks_for(cnt)), dim3(256), 0, s, (int)nch, cnt, part, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_update(int64_t n, int64_t nrhs, const double *V, int nv, const double *h, double *w,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_update_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, V, nv, h, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_normalize(int64_t n, int64_t nrhs, const double *w, const double *nrm2, double *v, hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_normalize_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, w, nrm2,
+                       v);
+    return hipGetLastError();
+}
+
+hipError_t launch_gm_combine(int64_t n, int64_t nrhs, const double *Z, int nz, const double *y, double *X, int64_t ldx,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(gm_combine_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, Z, nz, y, X,
+                       ldx);
     return hipGetLastError();
 }

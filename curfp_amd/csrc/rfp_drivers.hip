@@ -30,6 +30,7 @@
 #include <cmath>
 #include <cstring>
 #include <new>
+#include <vector>
 
 /* ------------------------------------------------------------------------ */
 /* Handle                                                                    */
@@ -66,6 +67,8 @@ extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
     ctx->d_info = nullptr;
     ctx->sweep_sync = nullptr;
     ctx->sweep_epoch = 0;
+    ctx->sweep_wait_limit = SWEEP_WAIT_LIMIT_DEFAULT;
+    ctx->fence = nullptr;
     ctx->prof = false;
     ctx->sh = nullptr;
     ctx->sh_bytes = 0;
@@ -92,6 +95,7 @@ extern "C" curfpStatus_t curfpDestroy(curfpHandle_t handle)
         (void)hipEventDestroy(r.e1);
     }
     for (hipEvent_t e : handle->prof_pool) (void)hipEventDestroy(e);
+    if (handle->fence) (void)hipEventDestroy(handle->fence);
     if (handle->ws) (void)hipFree(handle->ws);
     if (handle->sh) (void)hipFree(handle->sh);
     if (handle->d_info) (void)hipFree(handle->d_info);
@@ -177,10 +181,70 @@ extern "C" curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelCl
     return CURFP_STATUS_SUCCESS;
 }
 
+/* The handle's workspace, BF16 shadow, info word and sweep counters are
+ * reused by every call, so a stream switch must not let the new stream's
+ * work overtake what the old stream still has queued: the new stream waits
+ * on an event recorded on the old one (the stream being left must still be
+ * valid).  The reference allocates its workspace per call on the stream
+ * instead (src/curfp_spftrf.cpp:231-232, 288-291) and only stores the
+ * stream (src/curfp_handle.cpp:37-43). */
 extern "C" curfpStatus_t curfpSetStream(curfpHandle_t handle, hipStream_t stream)
 {
     RFP_CHECK_HANDLE(handle);
+    if (stream != handle->stream) {
+        if (!handle->fence && hipEventCreateWithFlags(&handle->fence, hipEventDisableTiming) != hipSuccess) {
+            handle->fence = nullptr;
+            return CURFP_STATUS_EXECUTION_FAILED;
+        }
+        if (hipEventRecord(handle->fence, handle->stream) != hipSuccess ||
+            hipStreamWaitEvent(stream, handle->fence, 0) != hipSuccess)
+            return CURFP_STATUS_EXECUTION_FAILED;
+    }
     handle->stream = stream;
+    return CURFP_STATUS_SUCCESS;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Device-side failures of asynchronous calls                                */
+/* ------------------------------------------------------------------------ */
+
+/* A substitution sweep that gave up waiting (or found its ticket counter
+ * shared with another launch) writes NaN and sets the error word in the
+ * handle's sync area.  Entry points that synchronise with the host anyway
+ * (spftrf's info, the refinement's norms, spfcon's records,
+ * curfpSynchronize) enqueue a copy of that word with their own read-back
+ * and, when it is set, clear the sync area and return EXECUTION_FAILED —
+ * the status the reference gives a runtime failure
+ * (src/curfp_internal.h:80-85). */
+static hipError_t sweep_err_fetch(curfpHandle_t h, int *host)
+{
+    *host = 0;
+    return hipMemcpyAsync(host, sweep_err_word(h->sweep_sync), sizeof(int), hipMemcpyDeviceToHost, h->stream);
+}
+
+static hipError_t sweep_err_check(curfpHandle_t h, int word)
+{
+    if (!word) return hipSuccess;
+    /* progress words, counters and the error word back to zero for the next sweep */
+    (void)launch_zero_f64((double *)h->sweep_sync, SWEEP_SYNC_BYTES / (int)sizeof(double), h->stream);
+    return hipErrorLaunchFailure;
+}
+
+extern "C" curfpStatus_t curfpSynchronize(curfpHandle_t handle)
+{
+    RFP_CHECK_HANDLE(handle);
+    int word = 0;
+    hipError_t e = sweep_err_fetch(handle, &word);
+    if (e == hipSuccess) e = hipStreamSynchronize(handle->stream);
+    if (e == hipSuccess) e = sweep_err_check(handle, word);
+    return st_of(e);
+}
+
+extern "C" curfpStatus_t curfpSetSweepWaitLimit(curfpHandle_t handle, int polls)
+{
+    RFP_CHECK_HANDLE(handle);
+    if (polls < 0) return CURFP_STATUS_INVALID_VALUE;
+    handle->sweep_wait_limit = polls;
     return CURFP_STATUS_SUCCESS;
 }
 
@@ -718,9 +782,11 @@ static curfpStatus_t pftrf_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     }
     e = f.potrf2(g.v11, sg.s11, g.v21, sg.s21, g.v22, sg.s22, g.n1, g.n2, 0, 0, s1);
     if (e != hipSuccess) return st_of(e);
-    int hinfo = INT_MAX;
+    int hinfo = INT_MAX, herr = 0;
     e = hipMemcpyAsync(&hinfo, h->d_info, sizeof(int), hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = sweep_err_fetch(h, &herr);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = sweep_err_check(h, herr);
     if (e != hipSuccess) return st_of(e);
     *info = (hinfo == INT_MAX) ? 0 : hinfo;
     return CURFP_STATUS_SUCCESS;
@@ -853,7 +919,8 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
             const unsigned long long ep = ++h->sweep_epoch;
             RFP_TRY(timed(h, CURFP_KERNEL_TRSV, fl, 0.0, [&] {
                 return launch_sweep_solve_f32((const float *)A, g, (const float *)inv, pm, (float *)B, bv,
-                                              (int)nrhs, dir == 1, bf, h->sweep_sync, ep, h->stream);
+                                              (int)nrhs, dir == 1, bf, h->sweep_sync, ep, h->sweep_wait_limit,
+                                              h->stream);
             }));
         }
         return hipSuccess;
@@ -944,9 +1011,12 @@ static curfpStatus_t pfcon_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     };
     PfconStats st;
     auto stats = [&](bool dot, int64_t j) -> hipError_t {
+        int herr = 0;
         RFP_TRY(pfcon_stats(x, dot ? isgn : nullptr, n, j, dst, s));
         RFP_TRY(hipMemcpyAsync(&st, dst, sizeof(st), hipMemcpyDeviceToHost, s));
-        return hipStreamSynchronize(s);
+        RFP_TRY(sweep_err_fetch(h, &herr));
+        RFP_TRY(hipStreamSynchronize(s));
+        return sweep_err_check(h, herr);
     };
     constexpr int ITMAX = 5;
     auto run = [&](T &est) -> hipError_t {
@@ -1311,7 +1381,8 @@ __global__ void residual_init_kernel(int64_t n, int64_t nrhs, double lambda, con
     const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= n * nrhs) return;
     const int64_t i = idx % n, c = idx / n;
-    R[i + c * ldr] = B[i + c * ldb] - lambda * X[i + c * ldx];
+    /* without B: R = lambda X, the first term of C X */
+    R[i + c * ldr] = B ? B[i + c * ldb] - lambda * X[i + c * ldx] : lambda * X[i + c * ldx];
 }
 
 static int64_t residual_splits(int64_t n) { return n >= 8192 ? 32 : 1; }
@@ -1321,8 +1392,8 @@ static size_t residual_scratch_bytes(int64_t n, int64_t k, int64_t nrhs)
     return (size_t)k * nrhs * (1 + residual_splits(n)) * sizeof(double);
 }
 
-/* R = B - (alpha op(A) op(A)^T + lambda I) X in FP64; `tmp` has
- * residual_scratch_bytes of scratch. */
+/* R = B - (alpha op(A) op(A)^T + lambda I) X in FP64, or R = C X when B is
+ * null; `tmp` has residual_scratch_bytes of scratch. */
 static hipError_t gen_residual_core(curfpHandle_t h, bool trans_n, int64_t n, int64_t k, int64_t nrhs, double alpha,
                                     const float *A, int64_t lda, double lambda, const double *X, int64_t ldx,
                                     const double *B, int64_t ldb, double *R, int64_t ldr, double *tmp)
@@ -1354,7 +1425,7 @@ static hipError_t gen_residual_core(curfpHandle_t h, bool trans_n, int64_t n, in
     g2.b = T; g2.vb = RfpView{0, k, 1};
     g2.c = R; g2.vc = RfpView{0, 1, ldr};
     g2.M = n; g2.N = nrhs; g2.K = k;
-    g2.alpha = -alpha; g2.beta = 1.0;
+    g2.alpha = B ? -alpha : alpha; g2.beta = 1.0;
     g2.a_f32 = true;
     return run_gemm<double>(h, g2, RFP_MATH_F64);
 }
@@ -1382,6 +1453,10 @@ extern "C" curfpStatus_t curfpSgenResidual(curfpHandle_t h, curfpOperation_t tra
 /* ------------------------------------------------------------------------ */
 
 static inline size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+/* Krylov steps per flexible-GMRES cycle of the refinement (fewer when the
+ * basis of 2m + 1 blocks of n x nrhs FP64 would pass 8 GiB) */
+constexpr int GMRES_RESTART = 20;
 
 extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo,
                                            curfpOperation_t trans, int n, int k, int nrhs, const double *alpha,
@@ -1412,13 +1487,14 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     const size_t r_bytes = align256((size_t)N * NR * sizeof(double));
     const size_t d_bytes = align256((size_t)N * NR * sizeof(float));
     void *ws = nullptr;
-    hipError_t e = rfp_workspace(h, shared + solve_bytes + r_bytes + d_bytes + 256, &ws);
+    hipError_t e = rfp_workspace(h, shared + solve_bytes + 2 * r_bytes + d_bytes + 256, &ws);
     if (e != hipSuccess) return st_of(e);
     char *base = (char *)ws;
     float *sv = (float *)(base + shared);
     double *R = (double *)(base + shared + solve_bytes);
-    float *D = (float *)(base + shared + solve_bytes + r_bytes);
-    double *sums = (double *)(base + shared + solve_bytes + r_bytes + d_bytes);  /* [0]=||R||^2 [1]=||X||^2 [2..3]=Gram */
+    double *Xbest = (double *)(base + shared + solve_bytes + r_bytes);   /* the iterate of smallest residual */
+    float *D = (float *)(base + shared + solve_bytes + 2 * r_bytes);
+    double *sums = (double *)(base + shared + solve_bytes + 2 * r_bytes + d_bytes);  /* [0]=||R||^2 [1]=||X||^2 [2..3]=Gram */
     /* BF16 mode: the factor is BF16-accurate, so the sweeps' off-diagonal
      * products run in BF16 too; the refinement absorbs their rounding */
     const bool bf16_solve = math_of(h) == RFP_MATH_BF16;
@@ -1460,32 +1536,195 @@ extern "C" curfpStatus_t curfpSpftrsRefine(curfpHandle_t h, curfpOperation_t tra
     if ((e = pftrs_core<float>(h, g, F, D, dv, NR, sv, false, bf16_solve)) != hipSuccess) return st_of(e);
     if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, false, s)) != hipSuccess) return st_of(e);
     it = 1;
-    double best_rn = INFINITY;
-    for (;;) {
-        /* R = B - C X (FP64), relative residual */
-        e = gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, X, ldx, B, ldb, R, N,
-                              (double *)base);
-        if (e == hipSuccess) e = launch_zero_f64(sums, 2, s);
-        if (e == hipSuccess) e = launch_sumsq(N, NR, R, N, sums, s);
-        if (e == hipSuccess) e = launch_sumsq(N, NR, X, ldx, sums + 1, s);
+
+    /* R = B - C X (FP64), ||R||_F, ||X||_F and the relative residual; the
+     * read-back also collects a failed sweep's error word */
+    double rn = INFINITY, nx = 0.0;
+    auto residual = [&]() -> hipError_t {
+        RFP_TRY(gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, X, ldx, B, ldb, R, N,
+                                  (double *)base));
+        RFP_TRY(launch_zero_f64(sums, 2, s));
+        RFP_TRY(launch_sumsq(N, NR, R, N, sums, s));
+        RFP_TRY(launch_sumsq(N, NR, X, ldx, sums + 1, s));
         double hs[2] = {0.0, 0.0};
-        if (e == hipSuccess) e = hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) return st_of(e);
-        const double nx = sqrt(hs[1]), rn = sqrt(hs[0]);
+        int herr = 0;
+        RFP_TRY(hipMemcpyAsync(hs, sums, sizeof(hs), hipMemcpyDeviceToHost, s));
+        RFP_TRY(sweep_err_fetch(h, &herr));
+        RFP_TRY(hipStreamSynchronize(s));
+        RFP_TRY(sweep_err_check(h, herr));
+        nx = sqrt(hs[1]);
+        rn = sqrt(hs[0]);
         rel = (normC > 0.0 && nx > 0.0) ? rn / (normC * nx) : rn;
+        return hipSuccess;
+    };
+    double best_rn = INFINITY, best_rel = INFINITY;
+    bool x_is_best = false;
+    auto track_best = [&]() {
+        x_is_best = rn < best_rn;
+        if (x_is_best) {
+            best_rn = rn;
+            best_rel = rel;
+        }
+    };
+
+    /* Classical refinement: X += F^{-1} fp32(R) while each step shrinks the
+     * residual at least fourfold.  A slower step means ||I - F^{-1} C|| is
+     * not small (a BF16 factor against a condition number of ~10^3 already
+     * diverges), and the correction equation C D = R is then solved by
+     * flexible GMRES preconditioned with F instead. */
+    bool krylov = false;
+    double prev_rn = INFINITY;
+    for (;;) {
+        if ((e = residual()) != hipSuccess) return st_of(e);
+        track_best();
         if (!(rel > tol) || it >= max_iter || !std::isfinite(rel)) break;
-        /* Refinement diverging (the factor is too inaccurate for this
-         * conditioning: ||I - F^{-1} C|| > 1): stop while X is still finite.
-         * The test uses the absolute residual — the relative one stays
-         * bounded while X and R grow together. */
-        if (it >= 3 && rn > 2.0 * best_rn) break;
-        best_rn = std::min(best_rn, rn);
+        if (it >= 2 && rn > 0.25 * prev_rn) {
+            krylov = true;
+            break;
+        }
+        prev_rn = rn;
+        if (x_is_best && (e = launch_copy_f64(N, NR, X, ldx, Xbest, N, s)) != hipSuccess) return st_of(e);
         /* D = F^{-1} fp32(R); X += D */
         if ((e = launch_cast_f64_f32(N, NR, R, N, D, N, s)) != hipSuccess) return st_of(e);
         if ((e = pftrs_core<float>(h, g, F, D, dv, NR, sv, true, bf16_solve)) != hipSuccess) return st_of(e);
         if ((e = launch_acc_f32_f64(N, NR, D, N, X, ldx, true, s)) != hipSuccess) return st_of(e);
         ++it;
+    }
+    /* the last classical step made the residual worse: continue from (or
+     * return) the best iterate */
+    if (!x_is_best && std::isfinite(best_rn)) {
+        if ((e = launch_copy_f64(N, NR, Xbest, N, X, ldx, s)) != hipSuccess) return st_of(e);
+        if (krylov) {
+            if ((e = residual()) != hipSuccess) return st_of(e);
+            track_best();
+        } else {
+            rel = best_rel;
+        }
+    }
+
+    if (krylov) {
+        /*
+         * Restarted flexible GMRES (Saad 1993) on C D = R, right-preconditioned
+         * by the FP32 factor, one Krylov space per right-hand side — the
+         * GMRES-IR of Carson & Higham (SIAM J. Sci. Comput. 40(2), 2018) with
+         * the preconditioned directions Z_j = F^{-1} V_j kept.  Because C Z_j
+         * is formed in FP64 from the generator and Z_j is stored exactly, the
+         * Arnoldi relation C Z = V H holds to FP64 rounding whatever the
+         * factor's accuracy, and the least-squares residual |g_{j+1}| of each
+         * column is its true FP64 residual.  Orthogonalisation: classical
+         * Gram-Schmidt applied twice.  Each step costs one FP32 solve (counted
+         * in *iters), one FP64 product with C and one small read-back.
+         */
+        const size_t blk = (size_t)N * NR;
+        int m = GMRES_RESTART;
+        while (m > 4 && (size_t)(2 * m + 1) * blk * sizeof(double) > ((size_t)8 << 30)) --m;
+        const int64_t nch = gm_chunks(N);
+        const size_t hd_elems = (size_t)(2 * (m + 1) + 1) * NR;
+        const size_t bytes = ((size_t)(2 * m + 1) * blk + (size_t)nch * (m + 1) * NR + hd_elems + (size_t)m * NR) *
+                             sizeof(double);
+        struct DevBuf {
+            void *p = nullptr;
+            ~DevBuf() { if (p) (void)hipFree(p); }
+        } kb;
+        if ((e = hipMalloc(&kb.p, bytes)) != hipSuccess) return st_of(e);
+        double *V = (double *)kb.p;                       /* m + 1 basis blocks */
+        double *Z = V + (size_t)(m + 1) * blk;            /* m preconditioned directions */
+        double *part = Z + (size_t)m * blk;
+        double *hd = part + (size_t)nch * (m + 1) * NR;   /* h (twice) and ||w||^2 of one step */
+        double *yd = hd + hd_elems;
+        double *h1 = hd, *h2 = hd + (size_t)(m + 1) * NR, *nrm2 = hd + (size_t)2 * (m + 1) * NR;
+        std::vector<double> H((size_t)(m + 1) * m * NR), cs((size_t)m * NR), sn((size_t)m * NR),
+            gv((size_t)(m + 1) * NR), y((size_t)m * NR), hh(hd_elems);
+        auto Hc = [&](int64_t c, int i, int j) -> double & { return H[((size_t)c * m + j) * (m + 1) + i]; };
+        double cycle_rn = rn;
+        while (rel > tol && it < max_iter && std::isfinite(rel)) {
+            /* V_0 = R / ||R|| per column */
+            int herr = 0;
+            if ((e = launch_gm_dots(N, NR, R, 1, R, part, nrm2, s)) != hipSuccess) return st_of(e);
+            if ((e = launch_gm_normalize(N, NR, R, nrm2, V, s)) != hipSuccess) return st_of(e);
+            if ((e = hipMemcpyAsync(hh.data() + 2 * (m + 1) * NR, nrm2, NR * sizeof(double), hipMemcpyDeviceToHost,
+                                    s)) != hipSuccess)
+                return st_of(e);
+            if ((e = hipStreamSynchronize(s)) != hipSuccess) return st_of(e);
+            std::fill(gv.begin(), gv.end(), 0.0);
+            for (int64_t c = 0; c < NR; ++c) gv[c] = sqrt(hh[2 * (m + 1) * NR + c]);
+            /* stop the cycle once the columns' residuals meet the outer test with a margin */
+            const double target = 0.5 * tol * normC * nx;
+            int j = 0;
+            while (j < m && it < max_iter) {
+                double *Vj = V + (size_t)j * blk, *Zj = Z + (size_t)j * blk, *W = R;
+                /* Z_j = F^{-1} V_j (FP32 solve), W = C Z_j (FP64) */
+                if ((e = launch_cast_f64_f32(N, NR, Vj, N, D, N, s)) != hipSuccess) return st_of(e);
+                if ((e = pftrs_core<float>(h, g, F, D, dv, NR, sv, true, bf16_solve)) != hipSuccess) return st_of(e);
+                if ((e = launch_acc_f32_f64(N, NR, D, N, Zj, N, false, s)) != hipSuccess) return st_of(e);
+                ++it;
+                if ((e = gen_residual_core(h, trans == CURFP_OP_N, N, K, NR, a, A, lda, lam, Zj, N, nullptr, N, W, N,
+                                           (double *)base)) != hipSuccess)
+                    return st_of(e);
+                /* W -= V h1; W -= V h2; V_{j+1} = W / ||W|| */
+                if ((e = launch_gm_dots(N, NR, V, j + 1, W, part, h1, s)) != hipSuccess) return st_of(e);
+                if ((e = launch_gm_update(N, NR, V, j + 1, h1, W, s)) != hipSuccess) return st_of(e);
+                if ((e = launch_gm_dots(N, NR, V, j + 1, W, part, h2, s)) != hipSuccess) return st_of(e);
+                if ((e = launch_gm_update(N, NR, V, j + 1, h2, W, s)) != hipSuccess) return st_of(e);
+                if ((e = launch_gm_dots(N, NR, W, 1, W, part, nrm2, s)) != hipSuccess) return st_of(e);
+                if ((e = launch_gm_normalize(N, NR, W, nrm2, V + (size_t)(j + 1) * blk, s)) != hipSuccess)
+                    return st_of(e);
+                if ((e = hipMemcpyAsync(hh.data(), hd, hd_elems * sizeof(double), hipMemcpyDeviceToHost, s)) !=
+                    hipSuccess)
+                    return st_of(e);
+                if ((e = sweep_err_fetch(h, &herr)) != hipSuccess) return st_of(e);
+                if ((e = hipStreamSynchronize(s)) != hipSuccess) return st_of(e);
+                if ((e = sweep_err_check(h, herr)) != hipSuccess) return st_of(e);
+                /* per column: new Hessenberg column, previous rotations, a new one */
+                double est2 = 0.0;
+                for (int64_t c = 0; c < NR; ++c) {
+                    for (int i = 0; i <= j; ++i) Hc(c, i, j) = hh[(size_t)i * NR + c] + hh[(size_t)(m + 1 + i) * NR + c];
+                    Hc(c, j + 1, j) = sqrt(hh[(size_t)2 * (m + 1) * NR + c]);
+                    for (int i = 0; i < j; ++i) {
+                        const double ci = cs[(size_t)i * NR + c], si = sn[(size_t)i * NR + c];
+                        const double t = ci * Hc(c, i, j) + si * Hc(c, i + 1, j);
+                        Hc(c, i + 1, j) = -si * Hc(c, i, j) + ci * Hc(c, i + 1, j);
+                        Hc(c, i, j) = t;
+                    }
+                    const double p = Hc(c, j, j), q = Hc(c, j + 1, j), r = std::hypot(p, q);
+                    const double cj = r > 0.0 ? p / r : 1.0, sj = r > 0.0 ? q / r : 0.0;
+                    cs[(size_t)j * NR + c] = cj;
+                    sn[(size_t)j * NR + c] = sj;
+                    Hc(c, j, j) = r;
+                    Hc(c, j + 1, j) = 0.0;
+                    gv[(size_t)(j + 1) * NR + c] = -sj * gv[(size_t)j * NR + c];
+                    gv[(size_t)j * NR + c] *= cj;
+                    est2 += gv[(size_t)(j + 1) * NR + c] * gv[(size_t)(j + 1) * NR + c];
+                }
+                ++j;
+                if (!(sqrt(est2) > target)) break;
+            }
+            /* y = H^{-1} g per column; X += Z y */
+            for (int64_t c = 0; c < NR; ++c)
+                for (int i = j - 1; i >= 0; --i) {
+                    double t = gv[(size_t)i * NR + c];
+                    for (int l = i + 1; l < j; ++l) t -= Hc(c, i, l) * y[(size_t)l * NR + c];
+                    const double d = Hc(c, i, i);
+                    y[(size_t)i * NR + c] = d != 0.0 ? t / d : 0.0;
+                }
+            if ((e = launch_copy_f64(N, NR, X, ldx, Xbest, N, s)) != hipSuccess) return st_of(e);
+            if ((e = hipMemcpyAsync(yd, y.data(), (size_t)j * NR * sizeof(double), hipMemcpyHostToDevice, s)) !=
+                hipSuccess)
+                return st_of(e);
+            if ((e = launch_gm_combine(N, NR, Z, j, yd, X, ldx, s)) != hipSuccess) return st_of(e);
+            if ((e = residual()) != hipSuccess) return st_of(e);
+            track_best();
+            /* a cycle that did not reduce the residual (rounding floor reached): keep the previous iterate */
+            if (!x_is_best) {
+                if ((e = launch_copy_f64(N, NR, Xbest, N, X, ldx, s)) != hipSuccess) return st_of(e);
+                rel = best_rel;
+                break;
+            }
+            if (!(rn < 0.9 * cycle_rn)) break;
+            cycle_rn = rn;
+        }
+        /* kb's buffer is freed after the work that reads it */
+        if ((e = hipStreamSynchronize(s)) != hipSuccess) return st_of(e);
     }
     if (iters) *iters = it;
     if (resid) *resid = rel;

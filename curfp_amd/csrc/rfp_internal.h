@@ -151,6 +151,8 @@ struct curfpContext {
     int            *d_info;    /* device int: first failing minor (INT_MAX = none) */
     void           *sweep_sync;   /* SWEEP_SYNC_BYTES of sweep-solve progress words */
     unsigned long long sweep_epoch;
+    int             sweep_wait_limit;  /* polls before a sweep workgroup's wait gives up */
+    hipEvent_t      fence;     /* orders the previous stream's work before the next stream's (curfpSetStream) */
     /* kernel profiling (curfpSetProfiling / curfpGetKernelStats) */
     bool            prof;
     std::vector<RfpProfRecord> prof_pending;
@@ -268,6 +270,10 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
  * the products written by launch_sweep_pmat_f32 at pm.
  * `sync` holds SWEEP_SYNC_BYTES of device words owned by the handle (zeroed
  * once at creation); `epoch` must grow by one per launch on that handle.
+ * A workgroup polls an earlier block's progress at most `wait_limit` times;
+ * when that runs out it stops waiting, writes its block as NaN and sets the
+ * error word (sweep_err_word), as does a workgroup whose ticket lies beyond
+ * the grid (a counter shared by overlapping launches).
  * Supports nrhs <= 64 * SWEEP_MAX_GROUPS.  bf16: the off-diagonal
  * contributions (all but the last two blocks' of each row) are BF16 products
  * with FP32 accumulation — for solves inside iterative refinement of a
@@ -277,7 +283,10 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
 #define SWEEP_SYNC_BYTES 1024
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
                                   RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
-                                  unsigned long long epoch, hipStream_t s);
+                                  unsigned long long epoch, int wait_limit, hipStream_t s);
+/* the error word inside the SWEEP_SYNC_BYTES area (non-zero after a failed sweep) */
+int *sweep_err_word(void *sync);
+#define SWEEP_WAIT_LIMIT_DEFAULT (1 << 20)
 /* C := alpha * A * Y + beta * C (trsv_block.hip) with A the n x n symmetric
  * matrix stored in RFP at A (geometry g), Y an n x N view and C an n x N view
  * (element (i, c) at off + i*rs + c*cs).  Y and C must not overlap.  FP32
@@ -308,6 +317,23 @@ hipError_t launch_cast_f64_f32(int64_t n, int64_t nrhs, const double *src, int64
 hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t ldd, double *X, int64_t ldx,
                               bool accumulate, hipStream_t s);
 hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s);
+hipError_t launch_copy_f64(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                           hipStream_t s);
+/* Flexible GMRES on n x nrhs FP64 blocks, one Krylov space per column (basis
+ * vector j at V + j*n*nrhs, leading dimension n):
+ *   gm_dots:      out[i*nrhs + c] = <V_i(:, c), w(:, c)> for i < nv, through
+ *                 `part` (gm_chunks(n) * nv * nrhs doubles), in a fixed order;
+ *   gm_update:    w(:, c) -= sum_i h[i*nrhs + c] V_i(:, c);
+ *   gm_normalize: v(:, c) = w(:, c) / sqrt(nrm2[c]);
+ *   gm_combine:   X(:, c) += sum_j y[j*nrhs + c] Z_j(:, c). */
+int64_t gm_chunks(int64_t n);
+hipError_t launch_gm_dots(int64_t n, int64_t nrhs, const double *V, int nv, const double *w, double *part,
+                          double *out, hipStream_t s);
+hipError_t launch_gm_update(int64_t n, int64_t nrhs, const double *V, int nv, const double *h, double *w,
+                            hipStream_t s);
+hipError_t launch_gm_normalize(int64_t n, int64_t nrhs, const double *w, const double *nrm2, double *v, hipStream_t s);
+hipError_t launch_gm_combine(int64_t n, int64_t nrhs, const double *Z, int nz, const double *y, double *X, int64_t ldx,
+                             hipStream_t s);
 /* out[0] += ||G||_F^2, out[1] += trace(G) for the symmetric k x k G given by
  * its lower triangle (column-major) */
 hipError_t launch_gram_norms(int64_t k, const double *G, double *out, hipStream_t s);

@@ -224,6 +224,7 @@ struct SweepArgs {
     unsigned *tick;               /* [0] ticket counter, [1] finished workgroups */
     unsigned long long epoch;
     int *err;
+    int wait_limit;               /* polls per wait before giving up */
 };
 
 /* global first row, view-local first row and row count of block b */
@@ -519,12 +520,11 @@ __global__ __launch_bounds__(256) void sweep_pmat_kernel(SweepArgs a)
             }
 }
 
-/* Bound on a workgroup's wait for an earlier block (2^20 polls of >= ~250 ns
- * each, a quarter of a second or more, against a few milliseconds for a whole
- * sweep), so that every wave exits even if the progress words were corrupted.
- * After a timeout the workgroup stops waiting, reports it in *err and writes
- * its block as NaN. */
-constexpr int SWEEP_SPIN_LIMIT = 1 << 20;
+/* A workgroup's wait for an earlier block is bounded by a.wait_limit polls
+ * (by default 2^20 polls of >= ~250 ns each, a quarter of a second or more,
+ * against a few milliseconds for a whole sweep), so that every wave exits
+ * even if the progress words were corrupted.  After a timeout the workgroup
+ * stops waiting, sets bit 0 of *err and writes its block as NaN. */
 
 /*
  * Forward (L Y = B) or backward (L^T X = Y) substitution over the whole
@@ -572,6 +572,12 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
     __syncthreads();
     /* wave-uniform values in scalar registers */
     const unsigned ticket = __builtin_amdgcn_readfirstlane(s_ticket);
+    if (ticket >= gridDim.x) {
+        /* the counter was shared with another launch: this block index would
+         * lie outside the factor, so touch nothing and report it (bit 1) */
+        if (tid == 0) __hip_atomic_fetch_or(a.err, 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
     const int o = (int)(ticket / (unsigned)a.groups), g = (int)(ticket % (unsigned)a.groups);
     const int b = TRANS ? a.nb - 1 - o : o;
     const int col0 = g * CW, cols = min(CW, a.nrhs - col0);
@@ -618,7 +624,7 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
             for (;;) {
                 v = __hip_atomic_load(a.prog + g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (v >= want) break;
-                if (++it > SWEEP_SPIN_LIMIT) {
+                if (++it > a.wait_limit) {
                     __hip_atomic_fetch_or(a.err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     s_fail = 1;
                     v = 0x7fffffffull;   /* no further waits */
@@ -1015,11 +1021,19 @@ hipError_t launch_sweep_pmat_f32(const float *L, const RfpGeom &g, const float *
     return hipGetLastError();
 }
 
+/* layout of the SWEEP_SYNC_BYTES area: SWEEP_MAX_GROUPS progress words,
+ * the ticket and finish counters, the error word */
+int *sweep_err_word(void *sync)
+{
+    return (int *)((unsigned *)((unsigned long long *)sync + SWEEP_MAX_GROUPS) + 2);
+}
+
 hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float *inv, const float *pm, float *B,
                                   RfpView bv, int nrhs, bool trans, bool bf16, void *sync,
-                                  unsigned long long epoch, hipStream_t s)
+                                  unsigned long long epoch, int wait_limit, hipStream_t s)
 {
     if (g.n <= 0 || nrhs <= 0) return hipSuccess;
+    if (wait_limit < 0) return hipErrorInvalidValue;
     const int cw = nrhs <= 16 ? 16 : nrhs <= 32 ? 32 : 64;
     const int groups = (nrhs + cw - 1) / cw;
     if (groups > SWEEP_MAX_GROUPS) return hipErrorInvalidValue;
@@ -1028,8 +1042,9 @@ hipError_t launch_sweep_solve_f32(const float *L, const RfpGeom &g, const float 
     a.nrhs = nrhs; a.groups = groups;
     a.prog = (unsigned long long *)sync;
     a.tick = (unsigned *)((unsigned long long *)sync + SWEEP_MAX_GROUPS);
-    a.err = (int *)(a.tick + 2);
+    a.err = sweep_err_word(sync);
     a.epoch = epoch;
+    a.wait_limit = wait_limit;
     const unsigned grid = (unsigned)a.nb * (unsigned)groups;
     const bool ycm = bv.rs == 1;
     /* the BF16 products are built for column-major right-hand sides only */

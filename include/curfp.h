@@ -80,11 +80,33 @@ typedef enum {
 
 typedef struct curfpContext *curfpHandle_t;
 
-/* Handle management — reference include/curfp.h:63-66, src/curfp_handle.cpp:4-51. */
+/* Handle management — reference include/curfp.h:63-66, src/curfp_handle.cpp:4-51.
+ * curfpSetStream differs in one respect: the handle's cached workspace is
+ * shared by all of its calls, so switching to a different stream makes the
+ * new stream wait for the work already queued on the old one (an event; the
+ * old stream must still be valid at the switch). */
 curfpStatus_t curfpCreate(curfpHandle_t *handle);
 curfpStatus_t curfpDestroy(curfpHandle_t handle);
 curfpStatus_t curfpSetStream(curfpHandle_t handle, hipStream_t stream);
 curfpStatus_t curfpGetStream(curfpHandle_t handle, hipStream_t *stream);
+
+/*
+ * curfpSynchronize — no reference counterpart.  Waits for the handle's
+ * stream and reports a failure that an earlier asynchronous call could only
+ * detect on the device: a substitution sweep of spftrs (FP32) whose bounded
+ * wait for an earlier block ran out, or whose block counter was shared with
+ * an overlapping launch, writes NaN into its right-hand sides and marks the
+ * handle.  This call, and every call that synchronises with the host anyway
+ * (spftrf, spfcon, curfpSpftrsRefine), then returns
+ * CURFP_STATUS_EXECUTION_FAILED once and clears the mark.
+ */
+curfpStatus_t curfpSynchronize(curfpHandle_t handle);
+
+/* Polls a sweep workgroup makes for an earlier block before it gives up
+ * (default 2^20, a quarter of a second or more; >= 0).  No reference
+ * counterpart: a bound that keeps a corrupted progress word from hanging the
+ * device. */
+curfpStatus_t curfpSetSweepWaitLimit(curfpHandle_t handle, int polls);
 
 /* Device memory currently held by the handle (cached workspace and the BF16
  * shadow of the factor); no reference counterpart. */
@@ -283,8 +305,11 @@ curfpStatus_t curfpGetKernelStats(curfpHandle_t handle, curfpKernelClass_t cls,
  * substitution sweeps take their off-diagonal products in BF16 (FP32
  * accumulation), matching the factor's BF16 accuracy; other modes solve in FP32.
  * B and X are FP64 column-major n x nrhs (ldb, ldx >= n); X need not be
- * initialised.  *iters = solves done, *resid = final relative residual.
- * Host-synchronous (reads the residual norm once per iteration).
+ * initialised.  *iters = solves done, *resid = relative residual of the
+ * returned X: the iterate of smallest residual (a step that increases the
+ * residual, a sign that the factor is too inaccurate for classical
+ * refinement, is undone, and the loop stops once the residual exceeds twice
+ * its best).  Host-synchronous (reads the residual norm once per iteration).
  */
 curfpStatus_t curfpSpftrsRefine(curfpHandle_t handle, curfpOperation_t transr,
                                 curfpFillMode_t uplo, curfpOperation_t trans,

@@ -180,24 +180,80 @@ def test_refinement_to_fp64_accuracy(curfp):
     assert info["rel_residual"] <= 1e-14 and err <= 1e-12, (info, err)
 
 
-def test_refinement_stops_when_diverging(curfp):
-    """A factor of C~ = A A^T + 0.01 I used to refine C = A A^T + 0.05 I with
-    rank(A A^T) < n: I - C~^{-1} C has eigenvalue -(0.05 - 0.01)/0.01 = -4 on
-    the null space of A^T, so the iteration diverges.  The driver stops after
-    a few solves with the iterate still finite and reports the residual."""
-    n, k, nrhs = 1200, 100, 4
-    A, B = problem(n, k, nrhs, seed=12)
+def _mismatched_factor(curfp, n, k, nrhs, seed, t="T", u="U"):
+    """The factor of C~ = A A^T + 0.01 I for the system C = A A^T + 0.05 I
+    with rank(A A^T) = k < n: I - C~^{-1} C has the eigenvalue
+    -(0.05 - 0.01)/0.01 = -4 on the null space of A^T, so classical
+    refinement diverges, while C~^{-1} C has its spectrum in [1, 5]."""
+    A, B = problem(n, k, nrhs, seed=seed)
     Ad = torch.from_numpy(A).cuda()
     C = torch.empty(n * (n + 1) // 2, device="cuda")
-    curfp.ssfrk(Ad, C)
-    curfp.add_to_diagonal(C, 0.01)
-    assert curfp.spftrf(C) == 0
+    curfp.ssfrk(Ad, C, transr=t, uplo=u)
+    curfp.add_to_diagonal(C, 0.01, transr=t, uplo=u)
+    assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    S = A.astype(np.float64) @ A.astype(np.float64).T + 0.05 * np.eye(n)
+    return A, B, Ad, C, S
+
+
+@pytest.mark.parametrize("t,u", VARIANTS)
+@pytest.mark.parametrize("nrhs", [1, 4, 33])
+def test_refinement_krylov_when_classical_diverges(curfp, t, u, nrhs):
+    """Classical refinement diverges on this pair (see _mismatched_factor); the
+    driver switches to flexible GMRES preconditioned by the factor and reaches
+    1e-6 and 1e-12, verified in float64 on the host, with the reported residual
+    matching the host's."""
+    n, k = 1200 if t == "T" else 1201, 100
+    A, B, Ad, C, S = _mismatched_factor(curfp, n, k, nrhs, seed=12 + nrhs, t=t, u=u)
     from curfp_amd import refine
 
-    X, info = refine.refine_solve(C, Ad, 0.05, torch.from_numpy(B).cuda(), tol=1e-6, max_iter=40)
-    assert info["solves"] <= 6, info
-    assert info["rel_residual"] > 1e-6
-    assert bool(torch.isfinite(X).all())
+    for tol in (1e-6, 1e-12):
+        X, info = refine.refine_solve(C, Ad, 0.05, torch.from_numpy(B).cuda(), tol=tol, max_iter=60, transr=t,
+                                      uplo=u)
+        r = O.dense_relative_residual(S, X.cpu().numpy(), B.astype(np.float64))
+        assert info["rel_residual"] <= tol and r <= 1.5 * tol, (tol, info, r)
+        assert abs(r - info["rel_residual"]) <= 0.05 * r + 1e-16
+        assert info["solves"] <= (30 if tol == 1e-6 else 50), info
+
+
+def test_refinement_returns_best_iterate(curfp):
+    """With too few solves to converge the driver returns its best iterate:
+    the residual it reports is the host's residual of the X it returns, and
+    no worse than that of the first solve alone."""
+    n, k, nrhs = 1200, 100, 4
+    A, B, Ad, C, S = _mismatched_factor(curfp, n, k, nrhs, seed=12)
+    from curfp_amd import refine
+
+    Bd = torch.from_numpy(B).cuda()
+    X1, i1 = refine.refine_solve(C, Ad, 0.05, Bd, tol=1e-6, max_iter=1)
+    r1 = O.dense_relative_residual(S, X1.cpu().numpy(), B.astype(np.float64))
+    for max_iter in (2, 3, 4, 6):
+        X, info = refine.refine_solve(C, Ad, 0.05, Bd, tol=1e-6, max_iter=max_iter)
+        r = O.dense_relative_residual(S, X.cpu().numpy(), B.astype(np.float64))
+        assert bool(torch.isfinite(X).all())
+        assert info["solves"] <= max_iter
+        assert abs(r - info["rel_residual"]) <= 0.05 * r + 1e-16, (max_iter, info, r)
+        assert r <= r1 * (1 + 1e-9), (max_iter, r, r1)
+
+
+def test_krr_solve_bf16_policy(curfp):
+    """krr_solve with math='bf16' and a refinement tolerance at conditionings
+    (kappa ~ 2.5e3 and 2.5e4) where classical refinement of a BF16 factor
+    stalls or the BF16 factor is not even positive definite: the result
+    reaches 1e-6 (flexible GMRES on the BF16 factor, or the split-BF16
+    fallback), checked in float64 on the host, and no LinAlgError escapes."""
+    from curfp_amd.autograd import krr_solve
+
+    n, k, nrhs = 2048, 128, 8
+    rng = np.random.default_rng(77)
+    A = (rng.standard_normal((n, k)) / math.sqrt(k)).astype(np.float32)
+    A = bf16_round(A).astype(np.float32)
+    Y = rng.standard_normal((n, nrhs)).astype(np.float32)
+    Ad, Yd = torch.from_numpy(A).cuda(), torch.from_numpy(Y).cuda()
+    for lam in (1e-2, 1e-3):
+        X = krr_solve(Ad, lam, Yd, math="bf16", refine_tol=1e-6, max_iter=60)
+        S = A.astype(np.float64) @ A.astype(np.float64).T + lam * np.eye(n)
+        r = O.dense_relative_residual(S, X.detach().cpu().numpy(), Y.astype(np.float64))
+        assert r <= 1.5e-6, (lam, r)
 
 
 @pytest.mark.parametrize("t,u", VARIANTS)

@@ -1,11 +1,15 @@
 """The reference's benchmark workloads at full size on one MI355X (parity at scale).
 
+  plain method, small: n = 2048 (T, U) and 2047 (N, L), k = 256, 64 rhs,
+             FP32 against the FP64 d* chain on the same inputs;
   plain method at scale: all 8 storage variants at n = 65536 and
-             65535 (k = 1024, lambda = 1, 64 rhs, FP32), the FP64 d* chain at
-             n = 32768, and a bit-exact strttf/stfttr round trip at n = 32768;
+             65535 (k = 1024, lambda = 1, 64 rhs, FP32), each against the
+             FP64 d* chain at the same n, the FP64 chain at n = 32768, and a
+             bit-exact strttf/stfttr round trip at n = 32768;
   precision: n = 90000, k = 4096, BF16 A (and its FP32 copy),
              lambda = 1e-2, 128 rhs: BF16, split-BF16 and FP32 factors, each
-             refined in FP64 to a relative residual <= 1e-6; forward error
+             refined in FP64 to a relative residual <= 1e-6 (the BF16 factor
+             through flexible GMRES once classical steps stall); forward error
              against an FP64 dpftrf/dpftrs solution at n = 16384;
   differentiation cost: forward + backward within 3x forward at
              n = 16384, k = 512, 16 rhs;
@@ -78,12 +82,39 @@ def timed(fn):
     return out, time.perf_counter() - t
 
 
+@pytest.mark.parametrize("n,t,u", [(2048, "T", "U"), (2047, "N", "L")])
+def test_small_vs_fp64(curfp, n, t, u):
+    """The small plain-method workload (k = 256, 64 rhs, lambda = 1): FP32
+    chain against the FP64 d* chain on the same inputs."""
+    k, nrhs, lam = 256, 64, 1.0
+    A, B = gen(n, k, nrhs, seed=0)
+    nc = norm_C(A, lam)
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(A, C, transr=t, uplo=u)
+    curfp.add_to_diagonal(C, lam, transr=t, uplo=u)
+    assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    X = B.clone()
+    curfp.spftrs(C, X, transr=t, uplo=u)
+    C64 = torch.empty(n * (n + 1) // 2, dtype=torch.float64, device="cuda")
+    curfp.dsfrk(A.double(), C64, transr=t, uplo=u)
+    curfp.add_to_diagonal(C64, lam, transr=t, uplo=u)
+    assert curfp.dpftrf(C64, transr=t, uplo=u) == 0
+    Xref = B.double()
+    curfp.dpftrs(C64, Xref, transr=t, uplo=u)
+    err = float((X.double() - Xref).norm() / Xref.norm())
+    r = rel_resid(curfp, A, lam, X, B, nc)
+    r64 = rel_resid(curfp, A, lam, Xref, B, nc)
+    record(f"c1_n{n}_{t}{u}", rel_residual=r, forward_error_vs_fp64=err, fp64_rel_residual=r64)
+    assert r <= 1e-6 and r64 <= 1e-14 and err <= 1e-5
+
+
 @pytest.mark.parametrize("n", [65536, 65535])
 def test_plain_all_variants_n65536(curfp, n):
     k, nrhs, lam = 1024, 64, 1.0
     A, B = gen(n, k, nrhs, seed=1)
     nc = norm_C(A, lam)
     C = torch.empty(n * (n + 1) // 2, device="cuda")
+    sols = {}
     for t in "NT":
         for u in "LU":
             X = B.clone()
@@ -97,8 +128,28 @@ def test_plain_all_variants_n65536(curfp, n):
 
             info, sec = timed(run)
             r = rel_resid(curfp, A, lam, X, B, nc)
+            sols[t + u] = X
             record(f"c2_n{n}_{t}{u}", seconds=sec, rel_residual=r, info=info)
             assert info == 0 and r <= 1e-6
+    del C
+    # forward error of every variant against the FP64 d* chain on the same inputs
+    C64 = torch.empty(n * (n + 1) // 2, dtype=torch.float64, device="cuda")
+    Xref = B.double()
+
+    def run64():
+        curfp.dsfrk(A.double(), C64)
+        curfp.add_to_diagonal(C64, lam)
+        info = curfp.dpftrf(C64, check=False)
+        curfp.dpftrs(C64, Xref)
+        return info
+
+    info64, sec64 = timed(run64)
+    assert info64 == 0
+    r64 = rel_resid(curfp, A, lam, Xref, B, nc)
+    errs = {v: float((X.double() - Xref).norm() / Xref.norm()) for v, X in sols.items()}
+    record(f"c2_n{n}_vs_fp64", fp64_seconds=sec64, fp64_rel_residual=r64, forward_error_vs_fp64=errs)
+    # kappa(C) ~ 5 here: the FP32 chain's forward error is a small multiple of n u
+    assert r64 <= 1e-14 and max(errs.values()) <= 1e-4
 
 
 def test_plain_fp64_and_pack_n32768(curfp):
@@ -161,8 +212,8 @@ def test_precision_n90000_k4096(curfp):
                      "solves": it["solves"], "rel_residual": r, "info": info}
         record(f"c3_n{n}_{mode}", **res[mode])
     del C
-    assert res["bf16x3"].get("rel_residual", 1.0) <= 1e-6
-    assert res["fp32"].get("rel_residual", 1.0) <= 1e-6
+    for mode in ("bf16", "bf16x3", "fp32"):
+        assert res[mode].get("rel_residual", 1.0) <= 1e-6, (mode, res[mode])
 
     # forward error against FP64 dpftrf / dpftrs at n = 16384, same distribution
     n2 = 16384

@@ -124,3 +124,59 @@ def test_reference_api_is_complete():
 
     missing = [f for f in REFERENCE_API if not callable(getattr(curfp, f, None))]
     assert not missing, missing
+
+
+REF_EXT = "/root/reference/curfp/_ext.cpp"
+
+
+def _reference_bindings():
+    """(functions {name: [keyword names]}, module constants, Handle methods)
+    as the reference module declares them (curfp/_ext.cpp:672-878)."""
+    text = open(REF_EXT).read()
+    text = text[text.index("PYBIND11_MODULE"):]
+    funcs = {}
+    for m in re.finditer(r'm\.def\("(\w+)",\s*&\w+\s*,(.*?)\);', text, re.S):
+        funcs[m.group(1)] = re.findall(r'py::arg\("(\w+)"\)', m.group(2))
+    attrs = re.findall(r'm\.attr\("(\w+)"\)', text)
+    methods = dict((m.group(1), re.findall(r'py::arg\("(\w+)"\)', m.group(2) or ""))
+                   for m in re.finditer(r'\.def\("(\w+)",\s*&Handle::\w+(.*?)(?=\n\s*\.def|\);)', text, re.S))
+    return funcs, attrs, methods
+
+
+def _keywords(fn):
+    sig = fn.__doc__.splitlines()[0]
+    return re.findall(r"(\w+):", sig[sig.index("(") + 1:sig.rindex(")")])
+
+
+@pytest.mark.skipif(not os.path.exists(REF_EXT), reason="the reference sources are not present")
+def test_extension_matches_reference_bindings():
+    """Every function of the reference's _curfp_C exists here under the same
+    name with the same keyword names in the same order, and every module
+    constant has the same value (curfp/_ext.cpp:684-878)."""
+    from curfp_amd import _curfp_C as ext
+
+    funcs, attrs, methods = _reference_bindings()
+    assert len(funcs) >= 26 and len(attrs) == 9
+    missing = [f for f in funcs if not hasattr(ext, f)]
+    assert not missing, missing
+    mismatched = {f: (_keywords(getattr(ext, f)), kw) for f, kw in funcs.items() if _keywords(getattr(ext, f)) != kw}
+    assert not mismatched, mismatched
+    values = {"OP_N": 0, "OP_T": 1, "FILL_LOWER": 0, "FILL_UPPER": 1, "NORM_MAX": 0, "NORM_ONE": 1, "NORM_FRO": 2,
+              "SIDE_LEFT": 0, "SIDE_RIGHT": 1}
+    assert sorted(attrs) == sorted(values)
+    assert {a: getattr(ext, a) for a in attrs} == values
+    for name, kw in methods.items():
+        assert hasattr(ext.Handle, name), name
+        if kw:
+            assert _keywords(getattr(ext.Handle, name))[1:] == kw, name
+
+
+def test_import_as_reference_name():
+    """``import curfp`` gives this package, with the extension at
+    curfp._curfp_C (the names the reference's code imports)."""
+    import curfp
+    import curfp_amd
+
+    assert curfp is curfp_amd
+    from curfp import _curfp_C, NORM_FRO, SIDE_RIGHT  # noqa: F401
+    assert _curfp_C is curfp_amd._curfp_C
