@@ -25,6 +25,7 @@
  * of being allocated per call (src/curfp_spftrf.cpp:231-232).
  */
 #include "rfp_internal.h"
+#include "trsm_panel_shape.h"
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -374,12 +375,6 @@ static RfpMath math_of(curfpHandle_t h)
     }
 }
 
-static inline int64_t leaf_split(int64_t m, int64_t leaf)
-{
-    const int64_t nb = (m + leaf - 1) / leaf;
-    return leaf * ((nb + 1) / 2);
-}
-
 /* ------------------------------------------------------------------------ */
 /* ssfrk                                                                     */
 /* ------------------------------------------------------------------------ */
@@ -670,7 +665,7 @@ struct Factor {
     hipError_t trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return hipSuccess;
-        if (!Prec<T>::f64 && m <= trsm_panel_max() && (!sh_hi || mb <= 16384)) {
+        if (!Prec<T>::f64 && m <= trsm_panel_max() && (!sh_hi || mb <= panel_shape::BF16_MAX_ROWS)) {
             /* FP32: the last levels of the recursion in one launch, each
              * workgroup substituting a slab of rows through all leaves.  In the
              * BF16 modes panels taller than 16384 rows keep the recursion,
@@ -719,22 +714,6 @@ struct Factor {
                       slot + m1 / leaf);
     }
 };
-
-/* BF16 shadow geometry: L11 (n1 x n1, ld1), L21 (n2 x n1, ld1), L22 (n2 x n2, ld2) */
-struct ShadowGeom {
-    RfpView s11, s21, s22;
-    int64_t elems;
-};
-static ShadowGeom shadow_geom(const RfpGeom &g)
-{
-    const int64_t ld1 = (g.n1 + 7) / 8 * 8, ld2 = (g.n2 + 7) / 8 * 8;
-    ShadowGeom sg;
-    sg.s11 = RfpView{0, ld1, 1};
-    sg.s21 = RfpView{g.n1 * ld1, ld1, 1};
-    sg.s22 = RfpView{(g.n1 + g.n2) * ld1, ld2, 1};
-    sg.elems = (g.n1 + g.n2) * ld1 + g.n2 * ld2;
-    return sg;
-}
 
 hipError_t rfp_shadow(curfpHandle_t h, size_t bytes, void **out)
 {

@@ -221,6 +221,30 @@ int gemm_inplace_limit(bool f64);
 /* Leaf size of the recursive factorisation / solves. */
 int rfp_leaf(bool f64);
 
+/* BF16 shadow geometry: L11 (n1 x n1, ld1), L21 (n2 x n1, ld1), L22 (n2 x n2, ld2) */
+struct ShadowGeom {
+    RfpView s11, s21, s22;
+    int64_t elems;
+};
+inline ShadowGeom shadow_geom(const RfpGeom &g)
+{
+    const int64_t ld1 = (g.n1 + 7) / 8 * 8, ld2 = (g.n2 + 7) / 8 * 8;
+    ShadowGeom sg;
+    sg.s11 = RfpView{0, ld1, 1};
+    sg.s21 = RfpView{g.n1 * ld1, ld1, 1};
+    sg.s22 = RfpView{(g.n1 + g.n2) * ld1, ld2, 1};
+    sg.elems = (g.n1 + g.n2) * ld1 + g.n2 * ld2;
+    return sg;
+}
+
+/* Split point of the recursive factorisation and triangular solves: the
+ * first ceil(blocks / 2) leaves of an m-row block. */
+inline int64_t leaf_split(int64_t m, int64_t leaf)
+{
+    const int64_t nb = (m + leaf - 1) / leaf;
+    return leaf * ((nb + 1) / 2);
+}
+
 /*
  * Diagonal tile Cholesky, in place on view v (m x m lower, m <= leaf).
  * On failure at local column j, atomicMin(info, info_base + j + 1).

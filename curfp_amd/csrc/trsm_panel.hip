@@ -22,14 +22,16 @@
  */
 #include "rfp_internal.h"
 #include "tile_stage.h"
+#include "trsm_panel_shape.h"
 
 namespace {
 
-constexpr int LF = TILE_LF;
-constexpr int MAXQ = 4;              /* leaves per panel */
-constexpr int RM = 32;               /* rows of X per workgroup */
-constexpr int XP = MAXQ * LF + 4;    /* LDS pitch of the slab (== 4 mod 64: conflict-free 16-byte reads) */
-constexpr int SP = LF + 4;           /* LDS pitch of a staged tile */
+using panel_shape::LF;
+using panel_shape::MAXQ;
+using panel_shape::RM;
+using panel_shape::XP;   /* == 4 mod 64: conflict-free 16-byte reads */
+using panel_shape::SP;
+static_assert(LF == TILE_LF && panel_shape::THREADS == 256, "panel shape");
 
 struct PanelArgs {
     float *a;                  /* RFP array */
@@ -180,7 +182,7 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
 
     /* slab: rows r0 .. r0 + rows, columns 0 .. m of X (zero beyond); all
      * loads issued before the LDS stores */
-    constexpr int SL = RM * MAXQ * LF / 4 / 256;   /* 16-byte pieces per thread */
+    constexpr int SL = panel_shape::SL;   /* 16-byte pieces per thread */
     const bool xc = p.X.cs == 1;
     {
         f32x4v v[SL];
