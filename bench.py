@@ -92,11 +92,12 @@ def run_fp32_step(curfp, A, C, B0, X, lam, ev=None):
     return info
 
 
-def run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, tol, max_iter, ev=None):
+def run_bf16_ir_step(curfp, refine, A, C, B0, lam, tol, max_iter, ev=None):
     if ev:
         ev[0].record()
     with curfp.math_mode("bf16"):
-        curfp.ssfrk(Abf, C)
+        # FP32 A: rounded to BF16 row blocks inside ssfrk (part of the step)
+        curfp.ssfrk(A, C)
         curfp.add_to_diagonal(C, lam)
         if ev:
             ev[1].record()
@@ -169,30 +170,152 @@ def roofline_of(entry: dict, pipe: str, steps: int, traffic: dict, cls: str = No
                         f"run after the timed steps"}
 
 
-def cpu_baseline(n, k, nrhs, seed=1):
-    """The oracle chain (numpy + scipy BLAS/LAPACK on the RFP sub-blocks, FP32)
-    on this host's cores, on a bounded sample n."""
+def copy_calibration(torch, args, dev) -> None:
+    """A device-to-device copy of args.copy_calib MiB: the kernel of known
+    traffic that scales rocprofv3's FETCH_SIZE / WRITE_SIZE counters."""
+    if args.copy_calib <= 0:
+        return
+    src = torch.ones(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+
+
+def host_cores() -> int:
+    """Cores this process may use: its affinity mask, capped by
+    OMP_NUM_THREADS when that is set (the host's share per GPU)."""
+    cores = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        cores = min(cores, int(env))
+    return cores
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(curfp, n, k, nrhs, seed=1, lam=1.0):
+    """LAPACK's own RFP routines (scipy.linalg.lapack ssfrk -> diagonal shift
+    -> spftrf -> spftrs, the call sequence of the reference's
+    python_tests/test_vs_lapack.py:44-90) on this host's cores, on the chain's
+    workload at a bounded n (scipy's LP64 wrappers index n(n+1)/2 in 32 bits,
+    n <= 46340): same distribution, transr='T', uplo='U', Fortran-ordered
+    buffers passed with overwrite_* so the wrappers copy nothing."""
     import numpy as np
-    from threadpoolctl import threadpool_info
+    from scipy.linalg import lapack
+    from threadpoolctl import threadpool_limits
 
-    from oracle import rfp_oracle as O
-
+    cores = host_cores()
     rng = np.random.default_rng(seed)
-    A = (rng.standard_normal((n, k)) / math.sqrt(k)).astype(np.float32)
-    B = rng.standard_normal((n, nrhs)).astype(np.float32)
-    t0 = time.perf_counter()
-    C = O.sfrk(n, k, 1.0, A, 0.0, np.zeros(n * (n + 1) // 2, np.float32), "T", "U", "N", exact=False)
-    C = O.add_to_diagonal(C, 1.0, "T", "U")
-    F, info = O.pftrf(n, C, "T", "U")
-    O.pftrs(n, F, B, "T", "U")
-    dt = time.perf_counter() - t0
-    threads = max([p.get("num_threads", 1) for p in threadpool_info() if p.get("internal_api") in
-                   ("openblas", "mkl", "blis")] or [1])
+    A = (rng.standard_normal((n, k), dtype=np.float32) / np.float32(math.sqrt(k))).astype(np.float32)
+    B = np.asfortranarray(rng.standard_normal((n, nrhs), dtype=np.float32))
+    C = np.zeros(n * (n + 1) // 2, np.float32)
+    diag = curfp.rfp_diag_indices(n, "T", "U", device="cpu").numpy()
+    with threadpool_limits(limits=cores):
+        t0 = time.perf_counter()
+        # A is row-major (n, k), i.e. the k x n column-major array of trans = 'T'
+        C = lapack.ssfrk(n, k, 1.0, A.T, 0.0, C, transr="T", uplo="U", trans="T", overwrite_c=1)
+        C[diag] += np.float32(lam)
+        t1 = time.perf_counter()
+        F, info = lapack.spftrf(n, C, transr="T", uplo="U", overwrite_a=1)
+        t2 = time.perf_counter()
+        X, info2 = lapack.spftrs(n, F, B, transr="T", uplo="U", overwrite_b=1)
+        t3 = time.perf_counter()
+    dt = t3 - t0
     fl = chain_flops(n, k, nrhs)["chain"]
-    return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": int(threads), "kind": "port",
-            "sample": f"oracle chain ssfrk+add_to_diagonal+spftrf+spftrs, n={n}, k={k}, nrhs={nrhs}, FP32, "
-                      f"{dt:.2f} s wall (numpy/scipy-openblas on the host)",
-            "seconds": dt, "info": int(info)}
+    return {"value": fl / dt / 1e12, "unit": "TFLOP/s", "cores": cores, "kind": "lapack",
+            "sample": f"LAPACK ssfrk+diagonal+spftrf+spftrs (scipy {__import__('scipy').__version__}, "
+                      f"OpenBLAS) on the chain's workload at n={n}, k={k}, nrhs={nrhs}, FP32, transr=T, uplo=U: "
+                      f"{dt:.2f} s wall on {cores} threads of {cpu_model()}",
+            "seconds": dt, "seconds_by_step": {"ssfrk+diagonal": t1 - t0, "spftrf": t2 - t1, "spftrs": t3 - t2},
+            "cpu_model": cpu_model(), "info": int(info or info2)}
+
+
+def run_pack(curfp, torch, args, dev) -> int:
+    """strttf / stfttr of a dense n-order upper triangle (row-major, lda = n)
+    and add_to_diagonal on an order-args.diag_n RFP matrix: kernel time per
+    call from the library's HIP events and wall time per call.  Algorithmic
+    bytes: 2 n(n+1)/2 * 4 per pack or unpack (each element of the triangle
+    read once and written once), 2 n * 4 per diagonal update."""
+    n = args.n
+    nt = n * (n + 1) // 2
+    g = torch.Generator(device=dev)
+    g.manual_seed(args.seed)
+    D = torch.randn(n, n, device=dev, generator=g)
+    arf = torch.empty(nt, device=dev)
+    U = torch.zeros(n, n, device=dev)
+    h = curfp._get_handle()
+    steps = max(args.steps, 1)
+    out = {}
+    for name, call in (("strttf", lambda: curfp.strttf_raw(h, curfp.OP_T, curfp.FILL_UPPER, n, D, n, arf)),
+                       ("stfttr", lambda: curfp.stfttr_raw(h, curfp.OP_T, curfp.FILL_UPPER, n, arf, U, n))):
+        for _ in range(max(args.warmup, 1)):
+            call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            call()
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / steps * 1e3
+        curfp.kernel_stats(reset=True)
+        curfp.set_profiling(True)
+        for _ in range(steps):
+            call()
+        st = curfp.kernel_stats(reset=True)["pack"]
+        curfp.set_profiling(False)
+        kms = st["ms"] / max(st["launches"], 1)
+        by = 2.0 * nt * 4
+        out[name] = {"wall_ms": wall, "kernel_ms": kms, "bytes": by, "GBps": by / (kms * 1e-3) / 1e9,
+                     "launches_per_call": st["launches"] / steps}
+        log(f"{name} n={n}: wall {wall:.3f} ms, kernel {kms:.3f} ms, {out[name]['GBps']:.0f} GB/s")
+    assert torch.equal(U, torch.triu(D)), "pack round trip is not bit-exact"
+    del D, U, arf
+    m = args.diag_n
+    C = torch.zeros(m * (m + 1) // 2, device=dev)
+    for _ in range(max(args.warmup, 1)):
+        curfp.add_to_diagonal(C, 1.0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        curfp.add_to_diagonal(C, 1.0)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps * 1e3
+    curfp.kernel_stats(reset=True)
+    curfp.set_profiling(True)
+    for _ in range(steps):
+        curfp.add_to_diagonal(C, 1.0)
+    st = curfp.kernel_stats(reset=True)["diag"]
+    curfp.set_profiling(False)
+    kms = st["ms"] / max(st["launches"], 1)
+    out["add_to_diagonal"] = {"n": m, "wall_ms": wall, "kernel_ms": kms, "bytes": 2.0 * m * 4,
+                              "GBps": 2.0 * m * 4 / (kms * 1e-3) / 1e9}
+    log(f"add_to_diagonal n={m}: wall {wall * 1e3:.1f} us, kernel {kms * 1e3:.1f} us, "
+        f"{out['add_to_diagonal']['GBps']:.1f} GB/s")
+    copy_calibration(torch, args, dev)
+    worst = min((out["strttf"], out["stfttr"]), key=lambda d: d["GBps"])
+    dom = "strttf" if worst is out["strttf"] else "stfttr"
+    line = {"metric": "rfp_pack_bandwidth", "value": worst["GBps"], "unit": "GB/s", "n_gpus": 1, "steps": steps,
+            "warmup": args.warmup, "ms_per_step": worst["kernel_ms"], "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic: torch.randn dense n x n",
+            "config": {"workload": f"strttf / stfttr of a dense upper triangle (transr=T, uplo=U), n={n}; "
+                                   f"add_to_diagonal at n={m}", "n": n, "diag_n": m},
+            "kernels": out,
+            "roofline": {"bound": "hbm", "kernel": f"rfp_copy_kernel ({dom}, the slower direction)",
+                         "achieved": worst["GBps"], "peak": 8000.0, "unit": "GB/s",
+                         "frac": worst["GBps"] / 8000.0, "traffic": None,
+                         "note": "2 n(n+1)/2 * 4 algorithmic bytes per call over the HIP-event kernel time"}}
+    print(json.dumps(line), flush=True)
+    return 0
 
 
 def run_norms(curfp, torch, args, dev) -> int:
@@ -431,15 +554,22 @@ def main() -> int:
     ap.add_argument("--nrhs", type=int, default=64)
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k", "pfcon"],
+    ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k", "pfcon",
+                                       "pack"],
                     default="both",
-                    help="norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
+                    help="pack: time strttf / stfttr at order n and add_to_diagonal at --diag-n; "
+                         "norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
                          "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor; "
                          "sfr2k: time ssfr2k (rank 2k, trans T) in the three math modes; "
                          "pfcon: time spfcon and ssfmv on the chain's matrix")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
-    ap.add_argument("--cpu-n", type=int, default=16384, help="CPU-baseline sample size (0 = skip)")
+    ap.add_argument("--cpu-n", type=int, default=32768,
+                    help="order of the LAPACK CPU-baseline sample (0 = skip; at most 46340)")
+    ap.add_argument("--diag-n", type=int, default=65536, help="order of the add_to_diagonal matrix (pack mode)")
+    ap.add_argument("--fp64-check", type=int, default=1,
+                    help="1: after timing, solve the same system with the FP64 d* chain and report each "
+                         "pipeline's forward error against it")
     ap.add_argument("--prof-steps", type=int, default=1,
                     help="extra steps run with per-launch HIP-event kernel statistics (roofline source)")
     ap.add_argument("--copy-calib", type=int, default=0,
@@ -467,6 +597,8 @@ def main() -> int:
         return run_pfcon(curfp, torch, args, dev)
     if args.mode == "pftri":
         return run_pftri(curfp, torch, args, dev)
+    if args.mode == "pack":
+        return run_pack(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
     fl = chain_flops(n, k, nrhs)
     log(f"inputs n={n} k={k} nrhs={nrhs} seed={args.seed}")
@@ -475,6 +607,7 @@ def main() -> int:
     C = torch.empty(n * (n + 1) // 2, dtype=torch.float32, device=dev)
     X = torch.empty_like(B0)
     result = {}
+    finals = {}
 
     def events():
         return [torch.cuda.Event(enable_timing=True) for _ in range(4)]
@@ -487,13 +620,12 @@ def main() -> int:
             except ImportError as exc:
                 result[pipe] = {"error": f"unavailable: {exc}"}
                 continue
-            Abf = A.to(torch.bfloat16)
         for w in range(args.warmup):
             log(f"{pipe} warmup {w}")
             if pipe == "fp32":
                 run_fp32_step(curfp, A, C, B0, X, lam)
             else:
-                run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, args.tol, args.max_iter)
+                run_bf16_ir_step(curfp, refine, A, C, B0, lam, args.tol, args.max_iter)
         torch.cuda.synchronize()
         per = []
         infos = []
@@ -505,7 +637,7 @@ def main() -> int:
                 infos.append(run_fp32_step(curfp, A, C, B0, X, lam, ev))
                 Xf = X
             else:
-                info, Xf, hist = run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, args.tol, args.max_iter, ev)
+                info, Xf, hist = run_bf16_ir_step(curfp, refine, A, C, B0, lam, args.tol, args.max_iter, ev)
                 infos.append(info)
             torch.cuda.synchronize()
             per.append([ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])])
@@ -525,7 +657,7 @@ def main() -> int:
                 if pipe == "fp32":
                     run_fp32_step(curfp, A, C, B0, X, lam)
                 else:
-                    run_bf16_ir_step(curfp, refine, A, Abf, C, B0, lam, args.tol, args.max_iter)
+                    run_bf16_ir_step(curfp, refine, A, C, B0, lam, args.tol, args.max_iter)
             torch.cuda.synchronize()
             prof_ms = (time.perf_counter() - tp) / args.prof_steps * 1e3
             stats = curfp.kernel_stats(reset=True)
@@ -550,7 +682,26 @@ def main() -> int:
         if hist is not None:
             entry["refinement_residuals"] = hist
         result[pipe] = entry
+        finals[pipe] = Xf.double().clone()
         log(f"{pipe}: {ms:.1f} ms/step, {entry['tflops']:.2f} TFLOP/s, resid {res:.3e}")
+
+    # forward error of each pipeline against the FP64 d* chain on the same inputs
+    if args.fp64_check and finals:
+        del C
+        C64 = torch.empty(n * (n + 1) // 2, dtype=torch.float64, device=dev)
+        Xref = B0.double()
+        curfp.dsfrk(A.double(), C64)
+        curfp.add_to_diagonal(C64, lam)
+        info64 = curfp.dpftrf(C64, check=False)
+        curfp.dpftrs(C64, Xref)
+        del C64
+        r64 = rel_residual(curfp, A, lam, Xref, B0.double(), normC)
+        for pipe, Xp in finals.items():
+            result[pipe]["forward_error_vs_fp64"] = float(torch.linalg.norm(Xp - Xref) / torch.linalg.norm(Xref))
+            log(f"{pipe}: forward error vs FP64 {result[pipe]['forward_error_vs_fp64']:.3e}")
+        fp64_ref = {"info": info64, "rel_residual": r64}
+    else:
+        fp64_ref = None
 
     # headline: fastest pipeline that meets the residual target
     ok = [p for p in pipelines if "ms_per_step" in result.get(p, {}) and result[p]["rel_residual"] <= args.tol]
@@ -562,17 +713,11 @@ def main() -> int:
     traffic = load_traffic()
     roofline = roofline_of(result.get(best, {}), best, args.prof_steps, traffic)
     roofline_fp32 = roofline_of(result.get("fp32", {}), "fp32", args.prof_steps, traffic, cls="gemm_f32")
-    if args.copy_calib > 0:
-        src = torch.ones(args.copy_calib * (1 << 20) // 4, dtype=torch.float32, device=dev)
-        dst = torch.empty_like(src)
-        torch.cuda.synchronize()
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        del src, dst
+    copy_calibration(torch, args, dev)
     cpu = None
     if args.cpu_n > 0:
         log(f"cpu baseline n={args.cpu_n}")
-        cpu = cpu_baseline(args.cpu_n, k, nrhs)
+        cpu = cpu_baseline(curfp, args.cpu_n, k, nrhs, lam=lam)
 
     line = {
         "metric": METRIC,
@@ -594,6 +739,8 @@ def main() -> int:
         "roofline": roofline,
         "roofline_fp32_path": roofline_fp32,
         "cpu_baseline": cpu,
+        "forward_error_vs_fp64": head.get("forward_error_vs_fp64"),
+        "fp64_reference": fp64_ref,
         "pipelines": result,
     }
     print(json.dumps(line), flush=True)
