@@ -564,7 +564,7 @@ def main() -> int:
                          "pfcon: time spfcon and ssfmv on the chain's matrix")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
-    ap.add_argument("--cpu-n", type=int, default=32768,
+    ap.add_argument("--cpu-n", type=int, default=46340,
                     help="order of the LAPACK CPU-baseline sample (0 = skip; at most 46340)")
     ap.add_argument("--diag-n", type=int, default=65536, help="order of the add_to_diagonal matrix (pack mode)")
     ap.add_argument("--fp64-check", type=int, default=1,
