@@ -30,6 +30,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <vector>
 
@@ -56,6 +57,8 @@ static curfpStatus_t st_of(hipError_t e)
         if (!(h)) return CURFP_STATUS_NOT_INITIALIZED;  \
     } while (0)
 
+static void noalloc_scratch_init(int device);
+
 extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
 {
     if (!handle) return CURFP_STATUS_INVALID_VALUE;
@@ -75,6 +78,7 @@ extern "C" curfpStatus_t curfpCreate(curfpHandle_t *handle)
     ctx->sh_bytes = 0;
     for (RfpKernelStat &st : ctx->prof_stats) st = RfpKernelStat{0, 0.0, 0.0, 0.0};
     if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return CURFP_STATUS_NOT_INITIALIZED; }
+    noalloc_scratch_init(ctx->device);
     if (hipMalloc((void **)&ctx->d_info, sizeof(int)) != hipSuccess) { delete ctx; return CURFP_STATUS_ALLOC_FAILED; }
     if (hipMalloc(&ctx->sweep_sync, SWEEP_SYNC_BYTES) != hipSuccess ||
         hipMemset(ctx->sweep_sync, 0, SWEEP_SYNC_BYTES) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess) {
@@ -873,7 +877,7 @@ static size_t pftrs_scratch_bytes(const RfpGeom &g, bool f64)
  * off-diagonal products in the FP32 sweeps (column-major B only). */
 template <typename T>
 static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B, RfpView bv, int64_t nrhs, T *inv,
-                             bool prepared = false, bool bf16 = false)
+                             bool prepared = false, bool bf16 = false, float *pm_at = nullptr)
 {
     const int64_t leaf = rfp_leaf(Prec<T>::f64);
     const int64_t s1 = (g.n1 + leaf - 1) / leaf;
@@ -884,8 +888,10 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
                       [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
     }
     if (!Prec<T>::f64 && nrhs <= 64 * SWEEP_MAX_GROUPS) {
-        /* FP32: one sweep launch per direction over all 128-row blocks */
-        float *pm = (float *)(inv + pftrs_inv_elems(g, false)) + ((size_t)8 << 20);
+        /* FP32: one sweep launch per direction over all 128-row blocks; the
+         * look-back products follow the leaf inverses and the split-K
+         * partials unless the caller places them (pm_at) */
+        float *pm = pm_at ? pm_at : (float *)(inv + pftrs_inv_elems(g, false)) + ((size_t)8 << 20);
         const double nb = (double)((g.n1 + leaf - 1) / leaf + (g.n2 + leaf - 1) / leaf);
         if (!prepared) {
             RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, 4.0 * nb * 2.0 * leaf * leaf * leaf, 0.0, [&] {
@@ -934,6 +940,72 @@ static curfpStatus_t pftrs_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     /* B(i, c): column-major n x nrhs (the reference ABI) or row-major */
     const RfpView bv = row_major ? RfpView{0, ldb, 1} : RfpView{0, 1, ldb};
     return st_of(pftrs_core<T>(h, g, A, B, bv, nrhs, (T *)ws));
+}
+
+/* ------------------------------------------------------------------------ */
+/* spftrs without device allocation                                          */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * For callers that cannot allocate (HIP graph capture, serving loops): the
+ * FP32 sweep solve with its scratch — the leaf inverses and the sweep's four
+ * look-back products per leaf — in one fixed device array instead of the
+ * handle's workspace.  At most NOALLOC_MAX_N rows: (n1/128 + n2/128) <= 128
+ * leaves, 5 * 128 * 128 * 128 floats = 40 MiB, held on every device the
+ * library is initialised on until the process ends.  Every call on a device
+ * shares that device's array, so calls that overlap on one GPU (two streams,
+ * two handles) race.  The sweep's progress words are the handle's own
+ * (allocated by curfpCreate); the producers of the inverses and products
+ * write their whole tiles, so nothing in the array needs clearing.
+ */
+constexpr int64_t NOALLOC_MAX_N = 16384;
+constexpr int64_t NOALLOC_MAX_LEAVES = NOALLOC_MAX_N / 128;
+constexpr size_t NOALLOC_SCRATCH_FLOATS = (size_t)5 * NOALLOC_MAX_LEAVES * 128 * 128;
+static __device__ __align__(256) float scratch[NOALLOC_SCRATCH_FLOATS];
+
+/* device address of `scratch` on each device, looked up once per device by
+ * curfpCreate with that device current (the host-side name is not a device
+ * address) */
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static float *g_scratch_ptr[NOALLOC_MAX_DEVICES];
+static bool g_scratch_done[NOALLOC_MAX_DEVICES];
+
+static void noalloc_scratch_init(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_done[device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_ptr[device] = e == hipSuccess ? (float *)ptr : nullptr;
+    g_scratch_done[device] = true;
+}
+
+static float *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_ptr[device];
+}
+
+extern "C" curfpStatus_t curfpSpftrsNoAlloc(curfpHandle_t h, curfpOperation_t transr, curfpFillMode_t uplo, int n,
+                                            int nrhs, const float *A, float *B, int ldb)
+{
+    RFP_CHECK_HANDLE(h);
+    if (n < 0 || nrhs < 0 || ldb < 1) return CURFP_STATUS_INVALID_VALUE;
+    if ((transr != CURFP_OP_N && transr != CURFP_OP_T) ||
+        (uplo != CURFP_FILL_MODE_LOWER && uplo != CURFP_FILL_MODE_UPPER))
+        return CURFP_STATUS_INVALID_VALUE;
+    if (n == 0 || nrhs == 0) return CURFP_STATUS_SUCCESS;
+    if (!A || !B || ldb < n) return CURFP_STATUS_INVALID_VALUE;
+    /* the sizes the fixed scratch holds, and the sweep path's width */
+    if (n > NOALLOC_MAX_N || nrhs > 64 * SWEEP_MAX_GROUPS) return CURFP_STATUS_INVALID_VALUE;
+    float *scr = noalloc_scratch(h->device);
+    if (!scr) return CURFP_STATUS_EXECUTION_FAILED;
+    const RfpGeom g = rfp_geom(n, transr == CURFP_OP_N, uplo == CURFP_FILL_MODE_LOWER);
+    float *pm = scr + pftrs_inv_elems(g, false);
+    return st_of(pftrs_core<float>(h, g, A, B, RfpView{0, 1, ldb}, nrhs, scr, false, false, pm));
 }
 
 /* ------------------------------------------------------------------------ */

@@ -320,6 +320,21 @@ curfpStatus_t curfpSpftrsRefine(curfpHandle_t handle, curfpOperation_t transr,
                                 int *iters, double *resid);
 
 /*
+ * curfpSpftrsNoAlloc — no counterpart in the reference.  curfpSpftrs for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc): the same FP32 sweep
+ * solve, same arguments and results, with its scratch (leaf inverses and
+ * look-back products) in a fixed 40 MiB per-device array that the library
+ * holds from its first curfpCreate on that device until the process ends;
+ * no allocation in the call.  Limits: n <= 16384 and nrhs <= 4096, else
+ * CURFP_STATUS_INVALID_VALUE.  All calls on one GPU share that array, so
+ * calls that overlap on one GPU (other streams or handles) race.
+ */
+curfpStatus_t curfpSpftrsNoAlloc(curfpHandle_t handle, curfpOperation_t transr,
+                                 curfpFillMode_t uplo, int n, int nrhs,
+                                 const float *A, float *B, int ldb);
+
+/*
  * curfpSgemmStrided / curfpDgemmStrided — the library's view GEMM on dense
  * strided operands (no reference counterpart; used by the autograd backward):
  *     C(i, j) = alpha * sum_k A(i, k) * B(j, k) + beta * C(i, j)
