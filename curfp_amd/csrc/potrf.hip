@@ -603,6 +603,93 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
 
 }  // namespace leaf32
 
+/* ------------------------------------------------------------------------ */
+/* FP64 leaf: two waves, column steps from an LDS-broadcast column           */
+/* ------------------------------------------------------------------------ */
+/*
+ * The 64 x 64 FP64 leaf.  Wave 0's lane i holds row i of the tile
+ * (a[k] = A(i, k)); wave 1's lane i holds column i of its inverse
+ * (a[k] = X(k, i), started from e_i).  Step j: wave 0 publishes column j
+ * (every lane's a[j]) in LDS, one barrier, and both waves read it back as
+ * broadcasts c[k] = A(k, j) and apply the same column step
+ *     a[j] /= L(j, j),   a[k] -= (a[j] / L(j, j)) c[k]   (k > j),
+ * which for a row is  L(i, j) = A(i, j) / L(j, j),  A(i, k) -= L(i, j) L(k, j)
+ * (right-looking Cholesky) and for an inverse column the column-oriented
+ * forward substitution of L x = e_i.  The column buffer is double-buffered,
+ * so one barrier per step orders both the publication and the reuse.  A
+ * ragged leaf (m < 64) is padded with the identity.  L(j, j) = sqrt of the
+ * pivot and 1 / L(j, j) are IEEE FP64 sqrt and division.
+ */
+namespace leaf64 {
+
+constexpr int NB = 64, P = NB + 1;
+
+__global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView v, int m, int *info,
+                                                           int info_base, double *inv)
+{
+    __shared__ double s[NB * P];
+    __shared__ __attribute__((aligned(16))) double W[2][NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    if (*info != INT_MAX) return;          /* an earlier leaf already failed */
+
+    /* the lower triangle into s, zero above, identity beyond m; loads run
+     * along the view's contiguous direction */
+    for (int e = 0; e < NB / 2; ++e) {
+        const int o = 2 * e + wave;        /* the other index, two per step */
+        const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
+        double val = r == c ? 1.0 : 0.0;
+        if (r < m && c < m) val = r >= c ? base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] : 0.0;
+        s[r * P + c] = val;
+    }
+    __syncthreads();
+    double a[NB];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) a[k] = wave == 0 ? s[lane * P + k] : (k == lane ? 1.0 : 0.0);
+
+    int fail = -1;
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        if (wave == 0) W[j & 1][lane] = a[j];
+        __syncthreads();
+        const double piv = W[j & 1][j];
+        if (!(piv > 0.0)) {                /* uniform: both waves read the same pivot */
+            fail = j;
+            break;
+        }
+        const double ljj = sqrt(piv), rinv = 1.0 / ljj;
+        const double aj = a[j];
+        a[j] = (wave == 0 && lane == j) ? ljj : aj * rinv;
+        const double t = a[j] * rinv;
+#pragma unroll
+        for (int k = j + 1; k < NB; ++k) a[k] = fma(-t, W[j & 1][k], a[k]);
+    }
+
+    /* L back to the view through s (coalesced), the inverse straight to its
+     * row-major slot (lane i writes column i); after a failure the partly
+     * factored tile is written back and info set, as LAPACK leaves it */
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) s[lane * P + k] = a[k];
+    }
+    __syncthreads();
+    for (int e = 0; e < NB / 2; ++e) {
+        const int o = 2 * e + wave;
+        const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
+        if (r < m && c < m && r >= c) base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] = s[r * P + c];
+    }
+    if (fail >= 0) {
+        if (tid == 0) atomicMin(info, info_base + fail + 1);
+        return;
+    }
+    if (inv && wave == 1) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) inv[k * NB + lane] = (k < m && lane < m) ? a[k] : 0.0;
+    }
+}
+
+}  // namespace leaf64
+
 template <typename T>
 static hipError_t potrf_tile(T *base, RfpView v, int m, int *info, int info_base, T *inv, hipStream_t s)
 {
@@ -613,7 +700,8 @@ static hipError_t potrf_tile(T *base, RfpView v, int m, int *info, int info_base
         hipLaunchKernelGGL(leaf32::potrf_leaf_kernel, dim3(1), dim3(256), 0, s, (float *)base, v, m, info, info_base,
                            (float *)inv);
     else
-        hipLaunchKernelGGL((potrf_tile_kernel<T, NB>), dim3(1), dim3(256), 0, s, base, v, m, info, info_base, inv);
+        hipLaunchKernelGGL(leaf64::potrf_leaf64_kernel, dim3(1), dim3(128), 0, s, (double *)base, v, m, info,
+                           info_base, (double *)inv);
     return hipGetLastError();
 }
 
