@@ -9,8 +9,9 @@
  * rectangle, v22 lower triangle) is copied by 64 x 64 tiles staged through
  * LDS: the source is read along its contiguous direction, the tile is
  * transposed in LDS when needed, and the destination is written along its
- * contiguous direction, so both sides stream coalesced.  One launch covers
- * all three blocks; all offsets are 64-bit.
+ * contiguous direction, so both sides stream coalesced — as 16-byte accesses
+ * when a view's offset and leading dimension keep them aligned.  One launch
+ * covers all three blocks; all offsets are 64-bit.
  */
 #include "rfp_internal.h"
 
@@ -23,6 +24,7 @@ struct CopySeg {
     int64_t rows, cols;   /* block shape */
     int64_t tiles_r, tiles_c, first;  /* tile grid and first workgroup id */
     int tri;              /* lower-triangular block */
+    int vsrc, vdst;       /* 16-byte accesses keep alignment on that side */
 };
 
 struct CopyParams {
@@ -44,10 +46,34 @@ __device__ __forceinline__ void seg_tile(const CopySeg &s, int64_t t, int64_t &t
     }
 }
 
+/* 16-byte vector of T that only assumes element alignment of its type */
+template <typename T> struct Vec16;
+template <> struct Vec16<float>  { typedef float  type __attribute__((ext_vector_type(4))); };
+template <> struct Vec16<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+/* Thread tid's pieces of a PT x PT tile along a view's contiguous direction:
+ * piece e covers VN consecutive elements starting at (li, lj) of the tile
+ * (rows li.. when the direction is i, columns lj.. otherwise).  VN = 1 gives
+ * the scalar layout (64 fast lanes x 4 slow rows). */
+template <int VN>
+struct TilePiece {
+    static constexpr int PER_ROW = PT / VN;            /* threads along the contiguous direction */
+    static constexpr int ROWS = 256 / PER_ROW;         /* slow rows per pass */
+    static constexpr int PASSES = PT / ROWS;
+    __device__ static void at(int tid, int e, bool along_i, int &li, int &lj)
+    {
+        const int c = (tid % PER_ROW) * VN, q = tid / PER_ROW + ROWS * e;
+        li = along_i ? c : q;
+        lj = along_i ? q : c;
+    }
+};
+
 template <typename T>
 __global__ __launch_bounds__(256) void rfp_copy_kernel(const T *src, T *dst, CopyParams p)
 {
     __shared__ T tile[PT][PT + 1];
+    constexpr int VN = 16 / sizeof(T);
+    typedef typename Vec16<T>::type V;
     const int64_t bid = blockIdx.x;
     int si = 0;
     while (si + 1 < p.nseg && bid >= p.seg[si + 1].first) ++si;
@@ -56,25 +82,93 @@ __global__ __launch_bounds__(256) void rfp_copy_kernel(const T *src, T *dst, Cop
     seg_tile(s, bid - s.first, ti, tj);
     const int64_t i0 = ti * PT, j0 = tj * PT;
     const int tid = threadIdx.x;
-    const int a = tid & 63, b = tid >> 6;   /* 64 fast lanes x 4 slow rows */
+    /* element (gi, gj) of the segment lies in it (the lower part for a triangle) */
+    auto inside = [&](int64_t gi, int64_t gj) { return gi < s.rows && gj < s.cols && (!s.tri || gi >= gj); };
 
-    /* read: fast lanes along the source's contiguous direction */
+    /* read along the source's contiguous direction; 16-byte loads when the
+     * view's offset and leading dimension keep every piece aligned (whole
+     * pieces only: pieces at the diagonal or the edge go element by
+     * element); all of a thread's loads are issued before its LDS stores */
     const bool src_i = s.src.rs == 1;
-#pragma unroll 4
-    for (int q = b; q < PT; q += 4) {
-        const int li = src_i ? a : q, lj = src_i ? q : a;
-        const int64_t gi = i0 + li, gj = j0 + lj;
-        T x = 0;
-        if (gi < s.rows && gj < s.cols && (!s.tri || gi >= gj)) x = src[s.src.at(gi, gj)];
-        tile[li][lj] = x;
+    if (s.vsrc) {
+        typedef TilePiece<VN> TP;
+        T x[TP::PASSES][VN];
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, src_i, li, lj);
+            const int64_t gi = i0 + li, gj = j0 + lj;
+            const int64_t gie = src_i ? gi + VN - 1 : gi, gje = src_i ? gj : gj + VN - 1;
+            const T *ptr = src + s.src.at(gi, gj);
+            if (inside(gie, gje) && inside(gi, gj) && (!s.tri || gi >= gje)) {
+                const V v = *(const V *)ptr;
+#pragma unroll
+                for (int c = 0; c < VN; ++c) x[e][c] = v[c];
+            } else {
+#pragma unroll
+                for (int c = 0; c < VN; ++c)
+                    x[e][c] = inside(src_i ? gi + c : gi, src_i ? gj : gj + c) ? ptr[c] : T(0);
+            }
+        }
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, src_i, li, lj);
+#pragma unroll
+            for (int c = 0; c < VN; ++c) tile[src_i ? li + c : li][src_i ? lj : lj + c] = x[e][c];
+        }
+    } else {
+        typedef TilePiece<1> TP;
+        T x[TP::PASSES];
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, src_i, li, lj);
+            const int64_t gi = i0 + li, gj = j0 + lj;
+            x[e] = inside(gi, gj) ? src[s.src.at(gi, gj)] : T(0);
+        }
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, src_i, li, lj);
+            tile[li][lj] = x[e];
+        }
     }
     __syncthreads();
+
+    /* write along the destination's contiguous direction, the same way */
     const bool dst_i = s.dst.rs == 1;
-#pragma unroll 4
-    for (int q = b; q < PT; q += 4) {
-        const int li = dst_i ? a : q, lj = dst_i ? q : a;
-        const int64_t gi = i0 + li, gj = j0 + lj;
-        if (gi < s.rows && gj < s.cols && (!s.tri || gi >= gj)) dst[s.dst.at(gi, gj)] = tile[li][lj];
+    if (s.vdst) {
+        typedef TilePiece<VN> TP;
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, dst_i, li, lj);
+            const int64_t gi = i0 + li, gj = j0 + lj;
+            const int64_t gie = dst_i ? gi + VN - 1 : gi, gje = dst_i ? gj : gj + VN - 1;
+            T *ptr = dst + s.dst.at(gi, gj);
+            if (inside(gie, gje) && inside(gi, gj) && (!s.tri || gi >= gje)) {
+                V v;
+#pragma unroll
+                for (int c = 0; c < VN; ++c) v[c] = tile[dst_i ? li + c : li][dst_i ? lj : lj + c];
+                *(V *)ptr = v;
+            } else {
+#pragma unroll
+                for (int c = 0; c < VN; ++c) {
+                    const int64_t ei = dst_i ? gi + c : gi, ej = dst_i ? gj : gj + c;
+                    if (inside(ei, ej)) ptr[c] = tile[dst_i ? li + c : li][dst_i ? lj : lj + c];
+                }
+            }
+        }
+    } else {
+        typedef TilePiece<1> TP;
+#pragma unroll
+        for (int e = 0; e < TP::PASSES; ++e) {
+            int li, lj;
+            TP::at(tid, e, dst_i, li, lj);
+            const int64_t gi = i0 + li, gj = j0 + lj;
+            if (inside(gi, gj)) dst[s.dst.at(gi, gj)] = tile[li][lj];
+        }
     }
 }
 
@@ -120,11 +214,24 @@ static int64_t copy_blocks(const CopyParams &p)
     return l.first + (l.tri ? l.tiles_r * (l.tiles_r + 1) / 2 : l.tiles_r * l.tiles_c);
 }
 
+/* a view's 16-byte pieces along its contiguous direction are aligned */
 template <typename T>
-static hipError_t rfp_copy(const T *src, T *dst, const CopyParams &p, hipStream_t s)
+static int vec_ok(const T *base, const RfpView &v)
+{
+    constexpr int64_t VN = 16 / sizeof(T);
+    const int64_t ld = v.rs == 1 ? v.cs : v.rs;
+    return ((uintptr_t)base % 16 == 0 && v.off % VN == 0 && ld % VN == 0) ? 1 : 0;
+}
+
+template <typename T>
+static hipError_t rfp_copy(const T *src, T *dst, CopyParams p, hipStream_t s)
 {
     const int64_t nb = copy_blocks(p);
     if (nb <= 0) return hipSuccess;
+    for (int k = 0; k < p.nseg; ++k) {
+        p.seg[k].vsrc = vec_ok(src, p.seg[k].src);
+        p.seg[k].vdst = vec_ok(dst, p.seg[k].dst);
+    }
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
     hipLaunchKernelGGL((rfp_copy_kernel<T>), dim3((unsigned)nb), dim3(256), 0, s, src, dst, p);
     return hipGetLastError();
