@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 BUILD = os.path.join(HERE, "_objs")
 
-HIP_SOURCES = ["gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "lansf.hip", "pfcon.hip", "rfp_drivers.hip"]
+HIP_SOURCES = ["gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "trsm_panel64.hip", "lansf.hip", "pfcon.hip", "rfp_drivers.hip"]
 LIB = os.path.join(HERE, "libcurfp_amd.so")
 
 HIPCC_FLAGS = [
@@ -35,6 +35,17 @@ HIPCC_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + INCLUDE,
 ]
+
+
+# Sources whose MFMA kernels keep their accumulators in VGPRs.  By default
+# the compiler places MFMA accumulators in AGPRs and, for accumulators that
+# live across a loop with branches in it (the guarded slice loads), copies
+# all of them AGPR -> VGPR -> AGPR on every iteration: the copies wait for the
+# MFMA pipeline to drain and the doubled registers cut occupancy (the FP64
+# 128 x 128 view GEMM: 336 registers per lane, 1 wave per SIMD; 218 and 2
+# waves in VGPR form).  Kernels whose accumulators exceed the 256
+# architectural VGPRs (the 256 x 256 BF16 tiles) keep the default.
+VGPR_FORM_SOURCES = {"gemm.hip", "potrf.hip", "trsm_panel.hip", "trsm_panel64.hip", "trsv_block.hip"}
 
 
 def ext_path() -> str:
@@ -53,7 +64,8 @@ def _run(cmd: list[str]) -> None:
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    extra = ["-mllvm", "-amdgpu-mfma-vgpr-form"] if src in VGPR_FORM_SOURCES else []
+    _run([os.path.join(ROCM, "bin", "hipcc"), *HIPCC_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
     return obj
 
 

@@ -682,6 +682,15 @@ struct Factor {
                                              sh_hi, sh_lo, Bs, math == RFP_MATH_BF16, h->stream);
             });
         }
+        if (Prec<T>::f64 && m <= trsm_panel64_max()) {
+            /* FP64: the same one-launch panel over 64-row leaves */
+            const double fl = (double)mb * (double)m * (double)m;
+            const double by = (double)mb * m * 8.0 * 2.0 + (double)m * m * 8.0;
+            return timed(h, CURFP_KERNEL_TRSM, fl, by, [&] {
+                return launch_trsm_panel_f64((double *)a, B, mb, L, m, (const double *)(inv + slot * leaf * leaf),
+                                             h->stream);
+            });
+        }
         if (m <= leaf)
             return gemm(B, a, B, inv + slot * leaf * leaf, RfpView{0, leaf, 1}, mb, m, m, 1.0, 0.0,
                         false, 1, RFP_MATH_F32, &Bs);

@@ -288,6 +288,16 @@ hipError_t launch_trsm_panel_f32(float *a, RfpView X, int64_t mb, RfpView L, int
                                  unsigned short *sh_hi, unsigned short *sh_lo, RfpView S, bool bf16, hipStream_t s);
 
 /*
+ * FP64 X := X * L^{-T} (trsm_panel64.hip) for X an mb x m view of the RFP
+ * array and L an m x m lower-triangular view with m <= trsm_panel64_max() (at
+ * most four 64-row leaves, whose inverses are at inv + q*64*64, row-major, zero
+ * above the diagonal); both views need a unit stride.
+ */
+int trsm_panel64_max();
+hipError_t launch_trsm_panel_f64(double *a, RfpView X, int64_t mb, RfpView L, int64_t m, const double *inv,
+                                 hipStream_t s);
+
+/*
  * FP32 substitution over the whole RFP factor in one launch (trsv_block.hip):
  * B (n x nrhs view) := L^{-1} B (trans = false) or L^{-T} B (trans = true),
  * given the inverses of the 128-row diagonal tiles of v11 then v22 at inv and

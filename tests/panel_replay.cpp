@@ -24,6 +24,10 @@
  *       workgroup is its first row r0 = 32 * blockIdx.x times the row stride
  *       plus a pattern that depends on blockIdx.x only through the row count
  *       of the last, ragged workgroup), or all workgroups with `all`
+ *   panel_replay chain64 n transr uplo [all]
+ *       the same for the FP64 panel kernel (trsm_panel64.hip) over every
+ *       panel launch of dpftrf at order n (64-row leaves, panels of at most
+ *       four leaves)
  *
  * Prints one summary line and exits 0, or prints the first violation and
  * exits 1.
@@ -181,6 +185,95 @@ struct Replay {
     }
 };
 
+/* FP64 panel (curfp_amd/csrc/trsm_panel64.hip, trsm_panel64_kernel): the
+ * tile and slab element maps are the kernel's own (panel64_shape::tile_elem,
+ * slab_elem); the masks are replicas of tile_load and the slab loops. */
+struct Replay64 : Replay {
+    void launch(const Launch &p)
+    {
+        namespace P = panel64_shape;
+        if (p.mb <= 0 || p.m <= 0) return;
+        ++launches;
+        const int m = (int)p.m, mb = (int)p.mb;
+        const int nq = (m + P::LF - 1) / P::LF, nsteps = nq * (nq + 1) / 2;
+        const int64_t grid = (p.mb + P::RM - 1) / P::RM;
+        for (int j = 0; j < nsteps; ++j) {
+            int q = 0, pp = j;                           /* step_of */
+            while (pp > q) { pp -= q + 1; ++q; }
+            const int mq = std::min(P::LF, m - q * P::LF);
+            const RfpView t = p.L.sub((int64_t)q * P::LF, (int64_t)pp * P::LF);
+            for (int tid = 0; tid < P::THREADS; ++tid)
+                for (int e = 0; e < P::TE; ++e) {
+                    int c, o;
+                    P::tile_elem(tid, e, c, o);
+                    coord(c >= 0 && c < P::LF && o >= 0 && o < P::LF, "tile element outside the tile", o, c);
+                    if (pp == q) {
+                        if (!(o < mq && c < mq)) continue;
+                        at(inv, (p.inv_slot + q) * (int64_t)P::LF * P::LF + o * P::LF + c, "leaf inverse");
+                    } else {
+                        const bool kc = p.L.cs == 1;
+                        const int n = kc ? o : c, k = kc ? c : o;
+                        if (!(n < mq)) continue;
+                        const int64_t li = (int64_t)q * P::LF + n, lk = (int64_t)pp * P::LF + k;
+                        coord(li < m && lk < li, "L tile outside the strict lower triangle", li, lk);
+                        at(arr, t.at(n, k), "L tile");
+                    }
+                }
+        }
+        std::vector<int64_t> wgs;
+        if (all_wg) {
+            for (int64_t b = 0; b < grid; ++b) wgs.push_back(b);
+        } else {
+            for (int64_t b : {(int64_t)0, (int64_t)1, grid - 2, grid - 1})
+                if (b >= 0 && b < grid && (wgs.empty() || wgs.back() < b)) wgs.push_back(b);
+        }
+        const bool xc = p.X.cs == 1;
+        for (int64_t b : wgs) {
+            const int64_t r0 = b * P::RM;
+            /* slab load and store: the same elements */
+            for (int tid = 0; tid < P::THREADS; ++tid)
+                for (int e = 0; e < P::SE; ++e) {
+                    int r, c;
+                    P::slab_elem(tid, e, xc, r, c);
+                    coord(r >= 0 && r < P::RM && c >= 0 && c < P::MAXQ * P::LF, "slab element outside LDS", r, c);
+                    if (!(r0 + r < mb && c < m)) continue;
+                    at(arr, p.X.at(r0 + r, c), "X slab");
+                }
+        }
+    }
+};
+
+/* the panel launches of rfp_drivers.hip Factor<double> (64-row leaves, every
+ * trsm_rt with m <= 256 is one panel launch) */
+struct Chain64 {
+    Replay64 &rp;
+    void trsm_rt(RfpView B, int64_t mb, RfpView L, int64_t m, int64_t slot)
+    {
+        const int64_t leaf = panel64_shape::LF;
+        if (m <= 0 || mb <= 0) return;
+        if (m <= panel64_shape::MAXQ * leaf) {
+            rp.launch(Launch{B, L, RfpView{0, 0, 0}, mb, m, slot});
+            return;
+        }
+        const int64_t m1 = leaf_split(m, leaf);
+        trsm_rt(B, mb, L, m1, slot);
+        trsm_rt(B.sub(0, m1), mb, L.sub(m1, m1), m - m1, slot + m1 / leaf);
+    }
+    void potrf2(RfpView V11, RfpView V21, RfpView V22, int64_t m1, int64_t m2, int64_t slot1, int64_t slot2)
+    {
+        potrf(V11, m1, slot1);
+        trsm_rt(V21, m2, V11, m1, slot1);
+        potrf(V22, m2, slot2);
+    }
+    void potrf(RfpView V, int64_t m, int64_t slot)
+    {
+        const int64_t leaf = panel64_shape::LF;
+        if (m <= leaf) return;
+        const int64_t m1 = leaf_split(m, leaf);
+        potrf2(V, V.sub(m1, 0), V.sub(m1, m1), m1, m - m1, slot, slot + m1 / leaf);
+    }
+};
+
 /* the panel launches of rfp_drivers.hip Factor<float>::potrf2 / potrf /
  * trsm_rt (the GEMM and leaf steps in between launch other kernels) */
 struct Chain {
@@ -219,7 +312,10 @@ struct Chain {
 
 int main(int argc, char **argv)
 {
-    if (argc < 2) { std::fprintf(stderr, "usage: panel_replay probe [rows] | chain n transr uplo bf16 [all]\n"); return 2; }
+    if (argc < 2) {
+        std::fprintf(stderr, "usage: panel_replay probe [rows] | chain n transr uplo bf16 [all] | chain64 n transr uplo [all]\n");
+        return 2;
+    }
     Replay rp;
     if (!std::strcmp(argv[1], "probe")) {
         const int64_t ld = 8192, m = 512, mb = 32768;
@@ -245,6 +341,21 @@ int main(int argc, char **argv)
         rp.where = "chain n=" + std::to_string(n) + " " + argv[3] + argv[4] + (rp.bf16 ? " bf16" : " fp32");
         Chain ch{rp, leaf, rp.bf16};
         ch.potrf2(g.v11, sg.s11, g.v21, sg.s21, g.v22, sg.s22, g.n1, g.n2, 0, s1);
+    } else if (!std::strcmp(argv[1], "chain64") && argc >= 5) {
+        const int64_t n = std::atoll(argv[2]);
+        const bool tn = argv[3][0] == 'N', lower = argv[4][0] == 'L';
+        Replay64 r64;
+        r64.all_wg = argc > 5 && !std::strcmp(argv[5], "all");
+        const RfpGeom g = rfp_geom(n, tn, lower);
+        const int64_t leaf = panel64_shape::LF;
+        const int64_t s1 = (g.n1 + leaf - 1) / leaf, s2 = (g.n2 + leaf - 1) / leaf;
+        r64.arr = Bounds{"RFP array", 0, n * (n + 1) / 2};
+        r64.inv = Bounds{"inverses", 0, (s1 + s2) * leaf * leaf};
+        r64.shadow = Bounds{"shadow", 0, 0};
+        r64.where = "chain64 n=" + std::to_string(n) + " " + argv[3] + argv[4];
+        Chain64 ch{r64};
+        ch.potrf2(g.v11, g.v21, g.v22, g.n1, g.n2, 0, s1);
+        rp = r64;
     } else {
         std::fprintf(stderr, "bad arguments\n");
         return 2;

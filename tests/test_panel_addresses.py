@@ -11,6 +11,8 @@
   the RFP array, the leaf-inverse slots and the BF16 shadow, X elements
   inside the launch's mb x m block and L reads inside the strict lower
   triangle.
+* The FP64 panel kernel (trsm_panel64.hip) over every panel launch of dpftrf
+  at n = 32768 and 32767 in the four layouts and at small ragged sizes.
 """
 import os
 import subprocess
@@ -65,3 +67,20 @@ def test_spftrf_panels_in_bounds(replay, n, t, u, bf16):
 def test_spftrf_panels_every_workgroup(replay, n, t, u, bf16):
     r = replay("chain", n, t, u, bf16, "all")
     assert r.returncode == 0, r.stdout
+
+
+@pytest.mark.parametrize("n", [32768, 32767])
+@pytest.mark.parametrize("t,u", [("N", "L"), ("N", "U"), ("T", "L"), ("T", "U")])
+def test_dpftrf_panels_in_bounds(replay, n, t, u):
+    """FP64 panel kernel (trsm_panel64.hip): every panel launch of dpftrf."""
+    r = replay("chain64", n, t, u)
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.startswith("ok ") and "832 launches" in r.stdout, r.stdout
+
+
+@pytest.mark.parametrize("n", [65, 300, 1025, 4097])
+@pytest.mark.parametrize("t,u", [("N", "U"), ("T", "L")])
+def test_dpftrf_panels_every_workgroup(replay, n, t, u):
+    r = replay("chain64", n, t, u, "all")
+    assert r.returncode == 0, r.stdout
+    assert r.stdout.startswith("ok "), r.stdout
