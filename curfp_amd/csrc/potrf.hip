@@ -403,29 +403,29 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
 #pragma unroll
     for (int k = 0; k < BS; ++k)
         v[k] = row_lane ? D[i * LP + k] : fmaxf(0.f, 1.f - fabsf(fi - (float)k));   /* row i, or e_i */
-    /* column jn from W half h into c (entries jn .. BS-1, in 16-byte groups) */
-    auto fetch = [&](float (&c)[BS], int jn, int h) {
-        const float4 *w = reinterpret_cast<const float4 *>(W + h * BS);
+    /* column jn from W half h into c (entries jn .. BS-1, in 16-byte groups
+     * held as vectors, so each group's LDS read lands in place) */
+    typedef float f32x4c __attribute__((ext_vector_type(4)));
+    auto fetch = [&](f32x4c (&c)[BS / 4], int jn, int h) {
+        const f32x4c *w = reinterpret_cast<const f32x4c *>(W + h * BS);
 #pragma unroll
-        for (int g = jn / 4; g < BS / 4; ++g) {
-            const float4 t = w[g];
-            c[4 * g] = t.x; c[4 * g + 1] = t.y; c[4 * g + 2] = t.z; c[4 * g + 3] = t.w;
-        }
+        for (int g = jn / 4; g < BS / 4; ++g) c[g] = w[g];
     };
-    float c[2][BS];
+    f32x4c c[2][BS / 4];
     if (row_lane) W[i] = v[0];
     wave_sync();
     fetch(c[0], 0, 0);
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
-        float (&cc)[BS] = c[j & 1];
-        const float piv = cc[j];
+        const f32x4c (&cv)[BS / 4] = c[j & 1];
+        auto cc = [&](int k) { return cv[k >> 2][k & 3]; };
+        const float piv = cc(j);
         if (!(piv > 0.f)) return j;
         const float rinv = __builtin_amdgcn_rsqf(piv);
         v[j] *= rinv;
         const float t = v[j] * rinv;
         if (j + 1 < BS) {
-            v[j + 1] = fmaf(-t, cc[j + 1], v[j + 1]);
+            v[j + 1] = fmaf(-t, cc(j + 1), v[j + 1]);
             /* half (j+1)&1 was last read at step j-1, whose reads this wave
              * has completed (they fed step j-1's updates) */
             if (row_lane) W[((j + 1) & 1) * BS + i] = v[j + 1];
@@ -433,7 +433,7 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
             fetch(c[(j + 1) & 1], j + 1, (j + 1) & 1);
         }
 #pragma unroll
-        for (int k = j + 2; k < BS; ++k) v[k] = fmaf(-t, cc[k], v[k]);
+        for (int k = j + 2; k < BS; ++k) v[k] = fmaf(-t, cc(k), v[k]);
     }
 #pragma unroll
     for (int k = 0; k < BS; ++k) {
