@@ -372,7 +372,8 @@ __device__ __forceinline__ void wave_sync()
  * Factor the 32 x 32 diagonal block at D (pitch LP) with one wave, write L
  * into D's lower triangle and inv(L) into Xd's lower triangle.  Lane i < 32
  * holds row i of the block; lane 32 + c holds column c of X = inv(L),
- * starting from e_c.  W is a 16-byte aligned LDS scratch of 2 x BS floats.
+ * starting from e_c.  W is a 16-byte aligned LDS scratch of 4 x BS floats
+ * (the upper half is written by the inverse lanes and never read).
  * Returns the failing local column or -1 (wave-uniform).
  *
  * Right-looking: column j is published by the row lanes in LDS and read
@@ -412,15 +413,23 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
         for (int g = jn / 4; g < BS / 4; ++g) c[g] = w[g];
     };
     f32x4c c[2][BS / 4];
-    if (row_lane) W[i] = v[0];
+    /* the inverse lanes' copies of the publication land in W[2 BS ..], which
+     * nothing reads: no divergent branch splits the column loop */
+    const int wpub = row_lane ? i : 2 * BS + i;
+    W[wpub] = v[0];
     wave_sync();
     fetch(c[0], 0, 0);
+    /* The pivot test is a select, not an exit, so the whole unrolled loop is
+     * one basic block and the trailing updates of column j can be scheduled
+     * into the LDS round trip of column j + 1.  After a failing pivot the
+     * arithmetic runs on (NaN / garbage, never read back). */
+    int fail = -1;
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
         const f32x4c (&cv)[BS / 4] = c[j & 1];
         auto cc = [&](int k) { return cv[k >> 2][k & 3]; };
         const float piv = cc(j);
-        if (!(piv > 0.f)) return j;
+        fail = (fail < 0 && !(piv > 0.f)) ? j : fail;
         const float rinv = __builtin_amdgcn_rsqf(piv);
         v[j] *= rinv;
         const float t = v[j] * rinv;
@@ -428,13 +437,15 @@ __device__ int factor_diag(float *D, float *Xd, float *W, int lane)
             v[j + 1] = fmaf(-t, cc(j + 1), v[j + 1]);
             /* half (j+1)&1 was last read at step j-1, whose reads this wave
              * has completed (they fed step j-1's updates) */
-            if (row_lane) W[((j + 1) & 1) * BS + i] = v[j + 1];
+            W[((j + 1) & 1) * BS + wpub] = v[j + 1];
             wave_sync();
             fetch(c[(j + 1) & 1], j + 1, (j + 1) & 1);
         }
 #pragma unroll
         for (int k = j + 2; k < BS; ++k) v[k] = fmaf(-t, cc(k), v[k]);
     }
+    fail = __builtin_amdgcn_readfirstlane(fail);
+    if (fail >= 0) return fail;
 #pragma unroll
     for (int k = 0; k < BS; ++k) {
         if (row_lane) D[i * LP + k] = v[k];
@@ -617,8 +628,8 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
  * (right-looking Cholesky) and for an inverse column the column-oriented
  * forward substitution of L x = e_i.  The column buffer is double-buffered,
  * so one barrier per step orders both the publication and the reuse.  A
- * ragged leaf (m < 64) is padded with the identity.  L(j, j) = sqrt of the
- * pivot and 1 / L(j, j) are IEEE FP64 sqrt and division.
+ * ragged leaf (m < 64) is padded with the identity.  1 / L(j, j) is the
+ * hardware reciprocal square root of the pivot refined by Newton steps.
  */
 namespace leaf64 {
 
@@ -656,7 +667,17 @@ __global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView
             fail = j;
             break;
         }
-        const double ljj = sqrt(piv), rinv = 1.0 / ljj;
+        /* 1 / sqrt(pivot): the hardware estimate and two Newton steps
+         * r += r (1 - piv r^2) / 2 (each doubles the correct bits); L(j, j) =
+         * piv r.  A few dependent FMAs instead of an IEEE sqrt and division on
+         * the column chain, to within an ulp or two of them. */
+        double rinv = __builtin_amdgcn_rsq(piv);
+#pragma unroll
+        for (int it = 0; it < 2; ++it) {
+            const double e = fma(-piv * rinv, rinv, 1.0);
+            rinv = fma(0.5 * rinv, e, rinv);
+        }
+        const double ljj = piv * rinv;
         const double aj = a[j];
         a[j] = (wave == 0 && lane == j) ? ljj : aj * rinv;
         const double t = a[j] * rinv;
