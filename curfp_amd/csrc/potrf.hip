@@ -657,16 +657,16 @@ __global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView
 #pragma unroll
     for (int k = 0; k < NB; ++k) a[k] = wave == 0 ? s[lane * P + k] : (k == lane ? 1.0 : 0.0);
 
+    /* The pivot test is a select (uniform: both waves read the same pivot),
+     * not an exit, so the column loop has no branches; after a failing pivot
+     * the arithmetic runs on, and only the columns before it are kept. */
     int fail = -1;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
         if (wave == 0) W[j & 1][lane] = a[j];
         __syncthreads();
         const double piv = W[j & 1][j];
-        if (!(piv > 0.0)) {                /* uniform: both waves read the same pivot */
-            fail = j;
-            break;
-        }
+        fail = (fail < 0 && !(piv > 0.0)) ? j : fail;
         /* 1 / sqrt(pivot): the hardware estimate and two Newton steps
          * r += r (1 - piv r^2) / 2 (each doubles the correct bits); L(j, j) =
          * piv r.  A few dependent FMAs instead of an IEEE sqrt and division on
@@ -686,18 +686,19 @@ __global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView
     }
 
     /* L back to the view through s (coalesced), the inverse straight to its
-     * row-major slot (lane i writes column i); after a failure the partly
-     * factored tile is written back and info set, as LAPACK leaves it */
+     * row-major slot (lane i writes column i); after a failure the columns
+     * factored before it are written back and info set */
     __syncthreads();
     if (wave == 0) {
 #pragma unroll
         for (int k = 0; k < NB; ++k) s[lane * P + k] = a[k];
     }
     __syncthreads();
+    const int cend = fail >= 0 ? fail : m;
     for (int e = 0; e < NB / 2; ++e) {
         const int o = 2 * e + wave;
         const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
-        if (r < m && c < m && r >= c) base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] = s[r * P + c];
+        if (r < m && c < cend && r >= c) base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] = s[r * P + c];
     }
     if (fail >= 0) {
         if (tid == 0) atomicMin(info, info_base + fail + 1);

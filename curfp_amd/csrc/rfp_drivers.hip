@@ -825,20 +825,19 @@ struct Solver {
 
     static int64_t solve_partial_elems() { return (int64_t)8 << 20; }
 
-    /* Size of the diagonal blocks solved in one kernel (FP32: 512-row block
-     * solves over 128-row leaf inverses; FP64: single leaves). */
-    int64_t block() const { return Prec<T>::f64 ? leaf : (int64_t)block_solve_rows(); }
+    /* Size of the diagonal blocks solved in one kernel: 512-row block solves
+     * over the leaf inverses (128-row FP32 leaves, 64-row FP64 leaves). */
+    int64_t block() const { return Prec<T>::f64 ? (int64_t)block_solve64_rows() : (int64_t)block_solve_rows(); }
 
     hipError_t leaf_solve(RfpView L, int64_t m, RfpView B, int64_t slot, bool trans)
     {
-        if (!Prec<T>::f64) {
-            return timed(h, CURFP_KERNEL_TRSV, 2.0 * (double)m * m * nrhs / 2.0, 0.0, [&] {
-                return launch_block_solve_f32((const float *)a, L, (int)m, (const float *)(inv + slot * leaf * leaf),
-                                              (float *)b, B, (int)nrhs, trans, h->stream);
-            });
-        }
-        const RfpView w = trans ? RfpView{0, 1, leaf} : RfpView{0, leaf, 1};
-        return gemm(B, inv + slot * leaf * leaf, w, b, B.t(), m, m, 1.0, 0.0, 2);
+        return timed(h, CURFP_KERNEL_TRSV, 2.0 * (double)m * m * nrhs / 2.0, 0.0, [&] {
+            if (Prec<T>::f64)
+                return launch_block_solve_f64((const double *)a, L, (int)m, (const double *)(inv + slot * leaf * leaf),
+                                              (double *)b, B, (int)nrhs, trans, h->stream);
+            return launch_block_solve_f32((const float *)a, L, (int)m, (const float *)(inv + slot * leaf * leaf),
+                                          (float *)b, B, (int)nrhs, trans, h->stream);
+        });
     }
 
     /* L Y = B in place, L lower (m x m), B (m x nrhs) */

@@ -275,6 +275,12 @@ int block_solve_rows();
 hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const float *inv, float *Bbase, RfpView Bv,
                                   int nrhs, bool trans, hipStream_t s);
 
+/* The FP64 twin (m <= block_solve64_rows(), 64-row leaves whose inverses are
+ * at inv + q*64*64, row-major, zero above the diagonal and beyond the leaf). */
+int block_solve64_rows();
+hipError_t launch_block_solve_f64(const double *Lbase, RfpView Lv, int m, const double *inv, double *Bbase,
+                                  RfpView Bv, int nrhs, bool trans, hipStream_t s);
+
 /*
  * FP32 X := X * L^{-T} (trsm_panel.hip) for X an mb x m view of the RFP array
  * and L an m x m lower-triangular view with m <= trsm_panel_max() (at most

@@ -193,6 +193,201 @@ __global__ __launch_bounds__(256) void block_solve_kernel(const float *Lbase, Rf
 
 
 /* ------------------------------------------------------------------------ */
+/* FP64 block solve                                                          */
+/* ------------------------------------------------------------------------ */
+/*
+ * The FP64 twin of block_solve_kernel: one workgroup per 16 right-hand-side
+ * columns keeps its (up to) 512 x 16 slice of B in LDS and walks the block's
+ * 64-row leaves in the same order — forward, for each leaf q, Y_q =
+ * inv(L_qq) B_q then B_r -= L_rq Y_q (r > q); backward, from the last leaf,
+ * X_q = inv(L_qq)^T B_q then B_r -= L_qr^T X_q (r < q) — each step one
+ * 64 x 64 operand (a leaf inverse or an off-diagonal tile of L, transposed
+ * for the backward solve) times the slice's 64 x 16 leaf on
+ * v_mfma_f64_16x16x4_f64.
+ *
+ * Wave w owns output rows 16 w .. 16 w + 15 of every leaf: its lanes load
+ * their own operand values straight from memory into the MFMA's A layout
+ * (no LDS staging), D steps ahead, and an update step writes only the
+ * wave's own rows of B_r while reading Y_q, which no update changes.  So the
+ * waves synchronise only around the inverse steps (all updates of B_q done
+ * before Y_q is formed from all of it, and Y_q complete before it is used),
+ * three barriers per leaf instead of two per step.  The k chunks alternate
+ * between two accumulators, so consecutive MFMAs are independent, and the
+ * zero half of a triangular inverse is skipped.
+ */
+namespace solve64 {
+
+constexpr int LF = 64;                   /* FP64 leaf */
+constexpr int MQ = 8;                    /* leaves per block: 512 rows */
+constexpr int CW = 16;                   /* right-hand-side columns per workgroup */
+constexpr int BP = CW + 1;               /* LDS pitch of the B slice (doubles) */
+constexpr int BPER = MQ * LF * CW / 256; /* B-slice elements per thread */
+constexpr int KC = LF / 4;               /* k chunks of one operand row */
+constexpr int D = 4;                     /* steps of operand loads in flight */
+
+typedef double f64x4v __attribute__((ext_vector_type(4)));
+
+/* operand A(i, k) = base[off + i*rs + k*cs] for i < vi, k < vk (zero beyond) */
+struct Op {
+    const double *base;
+    int64_t off, rs, cs;
+    int vi, vk;
+};
+
+/* element e of thread tid of the B slice as (row i, column c) */
+__device__ __forceinline__ void slice_elem(bool col_major, int tid, int e, int &i, int &c)
+{
+    const int idx = tid + 256 * e;
+    if (col_major) { i = idx % (MQ * LF); c = idx / (MQ * LF); }
+    else           { c = idx % CW;        i = idx / CW; }
+}
+
+template <bool TRANS>
+__global__ __launch_bounds__(256) void block_solve64_kernel(const double *Lbase, RfpView Lv, int m, const double *inv,
+                                                            double *Bbase, RfpView Bv, int nrhs)
+{
+    __shared__ double Bs[MQ * LF * BP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int c0 = blockIdx.x * CW;
+    const int nq = (m + LF - 1) / LF;
+    const int cols = min(CW, nrhs - c0);
+    const bool col_major = Bv.rs == 1;
+
+    /* B slice: rows [0, m), columns [c0, c0 + cols); zero beyond */
+    {
+        double v[BPER];
+#pragma unroll
+        for (int e = 0; e < BPER; ++e) {
+            int i, c;
+            slice_elem(col_major, tid, e, i, c);
+            v[e] = (i < m && c < cols) ? Bbase[Bv.off + (int64_t)i * Bv.rs + (int64_t)(c0 + c) * Bv.cs] : 0.0;
+        }
+#pragma unroll
+        for (int e = 0; e < BPER; ++e) {
+            int i, c;
+            slice_elem(col_major, tid, e, i, c);
+            Bs[i * BP + c] = v[e];
+        }
+    }
+
+    auto op_of = [&](int q, int r) {
+        const int mq = min(LF, m - q * LF), mr = min(LF, m - r * LF);
+        if (r == q) {
+            const double *W = inv + (int64_t)q * LF * LF;
+            return TRANS ? Op{W, 0, 1, LF, mq, mq} : Op{W, 0, LF, 1, mq, mq};
+        }
+        if (TRANS) {
+            /* A(i, k) = L(q*LF + k, r*LF + i) */
+            const RfpView v = Lv.sub((int64_t)q * LF, (int64_t)r * LF);
+            return Op{Lbase, v.off, v.cs, v.rs, mr, mq};
+        }
+        const RfpView v = Lv.sub((int64_t)r * LF, (int64_t)q * LF);
+        return Op{Lbase, v.off, v.rs, v.cs, mr, mq};
+    };
+
+    /* The lane's A values of a step: row ar = 16 w + lane % 16 of the operand,
+     * k = 4 c + lane / 16 for chunk c.  Every load is unconditional and used
+     * as loaded — an element outside the operand reads an exact zero of the
+     * first leaf inverse (row 0, column 63: above its diagonal), and the steps
+     * past the last re-load the last step's operand — so that each step waits
+     * for its own operand only. */
+    const int li = lane & 15, lk = lane >> 4;
+    const int ar = wave * 16 + li;
+    const double *const zero = inv + (LF - 1);
+    double x[D][KC];
+    int qs[D], rs[D];
+    auto load = [&](int d, const Op &o) {
+        const double *p = o.base + o.off + (int64_t)ar * o.rs + (int64_t)lk * o.cs;
+        const int64_t s4 = 4 * o.cs;
+        if (o.vi == LF && o.vk == LF) {
+            /* whole operand (all but a ragged last leaf): no masks */
+#pragma unroll
+            for (int c = 0; c < KC; ++c) x[d][c] = p[c * s4];
+            return;
+        }
+        const bool iok = ar < o.vi;
+#pragma unroll
+        for (int c = 0; c < KC; ++c) {
+            const double *ptr = (iok && 4 * c + lk < o.vk) ? p + c * s4 : zero;
+            x[d][c] = *ptr;
+        }
+    };
+    /* The step after (q, r) in solve order (the last step repeats).  Forward:
+     * for each leaf q, the inverse (q, q), then the updates r = q + 1 ..;
+     * backward: from the last leaf, (q, q), then r = 0 .. q - 1. */
+    auto next = [&](int &q, int &r) {
+        if (TRANS) {
+            if (r == q) { if (q > 0) r = 0; }
+            else if (r + 1 < q) ++r;
+            else { --q; r = q; }
+        } else {
+            if (r + 1 < nq) ++r;
+            else if (q + 1 < nq) { ++q; r = q; }
+        }
+    };
+
+    const int nsteps = nq + nq * (nq - 1) / 2;
+    /* (pq, pr): the step whose operand is loaded next (the first: the inverse
+     * of the first leaf in solve order) */
+    int pq = TRANS ? nq - 1 : 0, pr = pq;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        qs[d] = pq; rs[d] = pr;
+        load(d, op_of(pq, pr));
+        next(pq, pr);
+    }
+    for (int j0 = 0; j0 < nsteps; j0 += D) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int j = j0 + d;
+            if (j >= nsteps) break;
+            const int cq = qs[d], cr = rs[d];
+            const bool inv_step = cq == cr;
+            if (inv_step) lds_barrier();   /* every wave's updates of B_q are in */
+            /* acc = A (rows 16 w ..) * Bs[leaf cq] (64 x 16); a triangular
+             * inverse's zero 16-column chunks skipped (wave-uniform) */
+            int klo = 0, khi = LF;
+            if (inv_step) {
+                if (TRANS) klo = 16 * wave;
+                else       khi = 16 * (wave + 1);
+            }
+            const double *bcol = Bs + (cq * LF + lk) * BP + li;
+            f64x4v acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int c = 0; c < KC; c += 2) {
+                if (4 * c < klo || 4 * c >= khi) continue;
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[d][c], bcol[4 * c * BP], acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[d][c + 1], bcol[4 * (c + 1) * BP], acc1, 0, 0, 0);
+            }
+            qs[d] = pq; rs[d] = pr;
+            load(d, op_of(pq, pr));
+            next(pq, pr);
+            /* output row 16 w + lk + 4 rr of leaf cr, column li (the FP64
+             * MFMA layout): the wave's own rows */
+            double *dst = Bs + (cr * LF + wave * 16 + lk) * BP + li;
+            if (inv_step) {
+                lds_barrier();             /* every wave has read all of B_q */
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) dst[4 * rr * BP] = acc0[rr] + acc1[rr];
+                lds_barrier();             /* Y_q complete */
+            } else {
+#pragma unroll
+                for (int rr = 0; rr < 4; ++rr) dst[4 * rr * BP] -= acc0[rr] + acc1[rr];
+            }
+        }
+    }
+    lds_barrier();
+
+    for (int e = 0; e < BPER; ++e) {
+        int i, c;
+        slice_elem(col_major, tid, e, i, c);
+        if (i < m && c < cols) Bbase[Bv.off + (int64_t)i * Bv.rs + (int64_t)(c0 + c) * Bv.cs] = Bs[i * BP + c];
+    }
+}
+
+}  // namespace solve64
+
+/* ------------------------------------------------------------------------ */
 /* Whole-factor sweep: one launch per substitution direction                 */
 /* ------------------------------------------------------------------------ */
 
@@ -997,6 +1192,24 @@ hipError_t launch_block_solve_f32(const float *Lbase, RfpView Lv, int m, const f
     else
         hipLaunchKernelGGL((block_solve_kernel<false>), dim3(grid), dim3(256), 0, s, Lbase, Lv, m, inv, Bbase, Bv,
                            nrhs);
+    return hipGetLastError();
+}
+
+int block_solve64_rows() { return solve64::MQ * solve64::LF; }
+
+hipError_t launch_block_solve_f64(const double *Lbase, RfpView Lv, int m, const double *inv, double *Bbase,
+                                  RfpView Bv, int nrhs, bool trans, hipStream_t s)
+{
+    if (m <= 0 || nrhs <= 0) return hipSuccess;
+    if (m > block_solve64_rows()) return hipErrorInvalidValue;
+    if ((Lv.rs != 1 && Lv.cs != 1) || (Bv.rs != 1 && Bv.cs != 1)) return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)((nrhs + solve64::CW - 1) / solve64::CW);
+    if (trans)
+        hipLaunchKernelGGL((solve64::block_solve64_kernel<true>), dim3(grid), dim3(256), 0, s, Lbase, Lv, m, inv,
+                           Bbase, Bv, nrhs);
+    else
+        hipLaunchKernelGGL((solve64::block_solve64_kernel<false>), dim3(grid), dim3(256), 0, s, Lbase, Lv, m, inv,
+                           Bbase, Bv, nrhs);
     return hipGetLastError();
 }
 

@@ -252,6 +252,37 @@ def test_chain_residual_f64(curfp, n, t, u):
     assert res <= 1e-14, res
 
 
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 511, 512, 513, 1000, 1100])
+def test_dpftrs_block_geometry(curfp, n):
+    """The FP64 substitution's 512-row block solves over 64-row leaves (one
+    workgroup per 16 right-hand sides): sizes around the leaf and block
+    edges, partial column groups, two layouts, row-major B (Python) and
+    column-major B with ldb > n (C ABI), against float64 dense arithmetic;
+    repeated solves are bit-identical."""
+    k = 48
+    for nrhs in (1, 15, 16, 17, 64, 100):
+        A, B, S = _spd_problem(n, k, nrhs, seed=300 + n + nrhs, dtype=np.float64)
+        for t, u in (("T", "U"), ("N", "L")):
+            C = torch.empty(n * (n + 1) // 2, dtype=torch.float64, device="cuda")
+            curfp.dsfrk(dev(A), C, transr=t, uplo=u)
+            curfp.add_to_diagonal(C, 1.0, transr=t, uplo=u)
+            assert curfp.dpftrf(C, transr=t, uplo=u) == 0
+            X = dev(B)
+            curfp.dpftrs(C, X, transr=t, uplo=u)
+            assert O.dense_relative_residual(S, X.cpu().numpy(), B) <= 1e-14, (nrhs, t, u)
+            X2 = dev(B)
+            curfp.dpftrs(C, X2, transr=t, uplo=u)
+            assert torch.equal(X, X2), (nrhs, t, u)
+            ldb = n + 3
+            Bc = np.zeros((nrhs, ldb), np.float64)
+            Bc[:, :n] = B.T
+            Bt = dev(Bc)
+            curfp.dpftrs_raw(curfp._get_handle(), curfp.OP_T if t == "T" else curfp.OP_N,
+                             curfp.FILL_UPPER if u == "U" else curfp.FILL_LOWER, n, nrhs, C, Bt, ldb)
+            assert O.dense_relative_residual(S, Bt.cpu().numpy()[:, :n].T, B) <= 1e-14, (nrhs, t, u)
+            assert np.all(Bt.cpu().numpy()[:, n:] == 0)
+
+
 def test_spftrs_1d_and_column_major_abi(curfp):
     n, k = 301, 40
     A, B, S = _spd_problem(n, k, 5, seed=5)
