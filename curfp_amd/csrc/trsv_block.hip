@@ -981,11 +981,16 @@ __global__ __launch_bounds__(256) void sweep_solve_kernel(SweepArgs a)
         yv.load_lds(Ys, tid);
         yv.store(a, grow, rows, col0, cols, tid);
     }
-    /* publish: every wave's stores are written back before the progress word moves */
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    /* publish: every wave waits until its stores have reached L2; then one
+     * wave writes the L2 back (agent-scope release fence), waits for that
+     * write-back, and moves the progress word — a single write-back for the
+     * workgroup instead of one per wave */
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (tid == 0) {
-        __hip_atomic_store(a.prog + g, (a.epoch << 32) | (unsigned long long)(o + 1), __ATOMIC_RELEASE,
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        __hip_atomic_store(a.prog + g, (a.epoch << 32) | (unsigned long long)(o + 1), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         /* the last workgroup to finish re-arms the ticket counter for the next launch */
         if (__hip_atomic_fetch_add(a.tick + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
