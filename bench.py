@@ -399,6 +399,71 @@ def run_pftri(curfp, torch, args, dev) -> int:
     return 0
 
 
+def run_fp64(curfp, torch, args, dev) -> int:
+    """The FP64 d* chain of the plain-method-at-scale workload: dsfrk ->
+    add_to_diagonal -> dpftrf -> dpftrs at n = --fp64-n (default 32768),
+    k = --k, --nrhs right-hand sides, A and B the FP32 synthetic inputs held
+    in FP64; the same flop count as the FP32 chain, FP64 view GEMMs on the
+    matrix units (78.6 TFLOP/s peak)."""
+    n, k, nrhs, lam = args.fp64_n, args.k, args.nrhs, args.lam
+    fl = chain_flops(n, k, nrhs)
+    A32, B32 = make_inputs(n, k, nrhs, args.seed, dev)
+    A, B0 = A32.double(), B32.double()
+    normC = gen_norm_C(A32, lam)
+    C = torch.empty(n * (n + 1) // 2, dtype=torch.float64, device=dev)
+    X = torch.empty_like(B0)
+
+    def step():
+        X.copy_(B0)
+        curfp.dsfrk(A, C)
+        curfp.add_to_diagonal(C, lam)
+        info = curfp.dpftrf(C, check=False)
+        curfp.dpftrs(C, X)
+        return info
+
+    for _ in range(max(args.warmup, 0)):
+        step()
+    torch.cuda.synchronize()
+    times, info = [], 0
+    for s in range(max(args.steps, 1)):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        info = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f"fp64 step {s}: {times[-1] * 1e3:.1f} ms")
+    ms = sum(times) / len(times) * 1e3
+    stats = {}
+    if args.prof_steps > 0:
+        curfp.kernel_stats(reset=True)
+        curfp.set_profiling(True)
+        for _ in range(args.prof_steps):
+            step()
+        torch.cuda.synchronize()
+        stats = curfp.kernel_stats(reset=True)
+        curfp.set_profiling(False)
+    res = rel_residual(curfp, A32, lam, X, B0, normC)
+    tf = fl["chain"] / (ms * 1e-3) / 1e12
+    g = stats.get("gemm_f64", {"launches": 0, "ms": 0.0, "flops": 0.0})
+    roof = None
+    if g["launches"] and g["ms"] > 0:
+        ach = g["flops"] / (g["ms"] * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": ach, "peak": 78.6, "unit": "TFLOP/s", "frac": ach / 78.6,
+                "traffic": None, "class": "gemm_f64", "launches_per_step": g["launches"] / args.prof_steps,
+                "avg_launch_ms": g["ms"] / g["launches"], "flops_per_launch": g["flops"] / g["launches"]}
+    log(f"fp64 n={n}: {ms:.1f} ms/step, {tf:.2f} TFLOP/s, resid {res:.3e}")
+    line = {"metric": "dsfrk+dpftrf+dpftrs s & TFLOP/s (FP64 chain)", "value": tf, "unit": "TFLOP/s", "n_gpus": 1,
+            "steps": len(times), "warmup": args.warmup, "ms_per_step": ms, "seconds": ms * 1e-3,
+            "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+            "config": {"workload": f"FP64 RFP chain dsfrk+add_to_diagonal+dpftrf+dpftrs, n={n}",
+                       "n": n, "k": k, "nrhs": nrhs, "lambda": lam, "transr": "T", "uplo": "U"},
+            "rel_residual": res, "info": info,
+            "kernels": {kname: v for kname, v in stats.items() if v["launches"]},
+            "roofline": roof}
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def run_pfcon(curfp, torch, args, dev) -> int:
     """spfcon on the FP32 factor of C = A A^T + lam I (the chain's matrix):
     wall time per estimate (at most 7 forward + backward substitutions with
@@ -555,18 +620,20 @@ def main() -> int:
     ap.add_argument("--lam", type=float, default=1.0)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--mode", choices=["both", "fp32", "bf16_ir", "norms", "sfmm", "pftri", "sfr2k", "pfcon",
-                                       "pack"],
+                                       "pack", "fp64"],
                     default="both",
                     help="pack: time strttf / stfttr at order n and add_to_diagonal at --diag-n; "
                          "norms: time slansf (max / 1-norm / Frobenius) on an n-order RFP matrix instead; "
                          "sfmm: time ssfmm (side L, nrhs columns); pftri: time spftri on the chain's factor; "
                          "sfr2k: time ssfr2k (rank 2k, trans T) in the three math modes; "
-                         "pfcon: time spfcon and ssfmv on the chain's matrix")
+                         "pfcon: time spfcon and ssfmv on the chain's matrix; "
+                         "fp64: time the FP64 d* chain at --fp64-n")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-iter", type=int, default=30)
     ap.add_argument("--cpu-n", type=int, default=46340,
                     help="order of the LAPACK CPU-baseline sample (0 = skip; at most 46340)")
     ap.add_argument("--diag-n", type=int, default=65536, help="order of the add_to_diagonal matrix (pack mode)")
+    ap.add_argument("--fp64-n", type=int, default=32768, help="order of the FP64 chain (fp64 mode)")
     ap.add_argument("--fp64-check", type=int, default=1,
                     help="1: after timing, solve the same system with the FP64 d* chain and report each "
                          "pipeline's forward error against it")
@@ -597,6 +664,8 @@ def main() -> int:
         return run_pfcon(curfp, torch, args, dev)
     if args.mode == "pftri":
         return run_pftri(curfp, torch, args, dev)
+    if args.mode == "fp64":
+        return run_fp64(curfp, torch, args, dev)
     if args.mode == "pack":
         return run_pack(curfp, torch, args, dev)
     n, k, nrhs, lam = args.n, args.k, args.nrhs, args.lam
