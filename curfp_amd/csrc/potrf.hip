@@ -16,6 +16,7 @@
  */
 #include "rfp_internal.h"
 #include <climits>
+#include <type_traits>
 
 template <typename T> struct LeafSize;
 template <> struct LeafSize<float>  { static constexpr int value = 128; };
@@ -325,13 +326,16 @@ __device__ __forceinline__ void mma(f32x16_t &acc, const float *A, int ai, int a
                                     int lane)
 {
     const int r = lane & 31, h = lane >> 5;
-    /* all operands first: one LDS round trip for the 16 dependent MFMAs */
+    /* all operands first: one LDS round trip for the 16 dependent MFMAs (the
+     * scheduling barrier keeps the compiler from sinking each read to its
+     * MFMA, which would put a round trip before every MFMA) */
     float a[BS / 2], b[BS / 2];
 #pragma unroll
     for (int s = 0; s < BS / 2; ++s) {
         a[s] = A[r * ai + (2 * s + h) * ak];
         b[s] = B[(2 * s + h) * bk + r * bj];
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int s = 0; s < BS / 2; ++s) acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[s], b[s], acc, 0, 0, 0);
 }
@@ -480,93 +484,112 @@ __device__ __forceinline__ void inv_block(const float *L, float *X, float *Tw, i
     wave_sync();
 }
 
-__global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v, int m, int *info, int info_base,
-                                                         float *inv)
+/*
+ * The leaf in phases, shared by the one-leaf kernel and the two-leaf kernel
+ * below.  The factor lives in Lt and its inverse is built in Xt (both NB x LP
+ * LDS arrays); Tt is the waves' 32 x 33 scratch.
+ */
+
+constexpr int PER = NB * NB / 256;   /* tile elements per thread */
+
+/* Issue the loads of the lower triangle of the m x m tile (view v) into x.
+ * Only the lower triangle is read by the factorisation; an entry above the
+ * diagonal is loaded from the diagonal of its row instead (always inside the
+ * view) and its value is never used.  Rows / columns at or beyond m are
+ * clamped the same way (leaf_put replaces them by identity padding). */
+__device__ __forceinline__ void leaf_fetch(float (&x)[PER], const float *base, RfpView v, int m, int tid)
 {
-    __shared__ float L[NB * LP];
-    __shared__ float X[NB * LP];
-    __shared__ __attribute__((aligned(16))) float T[4][BS * 33];
-    __shared__ int fail;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    /* Load the tile (all loads in flight before the LDS stores, the check
-     * of info among them).  Only the lower triangle is read by the
-     * factorisation; an entry above the diagonal is loaded from the diagonal
-     * of its row instead (always inside the view) and its value is never
-     * used.  Rows / columns at or beyond m are clamped the same way and then
-     * replaced by identity padding. */
-    {
-        constexpr int PER = NB * NB / 256;
-        const int mc = m - 1;
-        const bool colv = v.rs == 1;
-        float x[PER];
-        if (colv) {
-            /* column view: lanes run down a column */
-            const int i = tid % NB, j0 = tid / NB, ic = min(i, mc);
-            const float *p = base + v.off + ic;
+    const int mc = m - 1;
+    if (v.rs == 1) {
+        /* column view: lanes run down a column */
+        const int i = tid % NB, j0 = tid / NB, ic = min(i, mc);
+        const float *p = base + v.off + ic;
 #pragma unroll
-            for (int e = 0; e < PER; ++e) x[e] = p[(int64_t)min(j0 + 2 * e, ic) * v.cs];
-        } else {
-            /* row view: lanes run along a row */
-            const int j = tid % NB, i0 = tid / NB;
-            const float *p = base + v.off;
+        for (int e = 0; e < PER; ++e) x[e] = p[(int64_t)min(j0 + 2 * e, ic) * v.cs];
+    } else {
+        /* row view: lanes run along a row */
+        const int j = tid % NB, i0 = tid / NB;
+        const float *p = base + v.off;
 #pragma unroll
-            for (int e = 0; e < PER; ++e) {
-                const int ic = min(i0 + 2 * e, mc);
-                x[e] = p[(int64_t)ic * v.rs + min(j, ic) * v.cs];
-            }
-        }
-        if (*info != INT_MAX) return;
-        if (colv) {
-            const int i = tid % NB, j0 = tid / NB;
-#pragma unroll
-            for (int e = 0; e < PER; ++e) L[i * LP + j0 + 2 * e] = x[e];
-        } else {
-            const int j = tid % NB, i0 = tid / NB;
-#pragma unroll
-            for (int e = 0; e < PER; ++e) L[(i0 + 2 * e) * LP + j] = x[e];
-        }
-        if (m < NB) {
-            __syncthreads();
-            for (int idx = tid; idx < NB * NB; idx += 256) {
-                const int i = idx / NB, j = idx % NB;
-                if (i >= m || j >= m) L[i * LP + j] = (i == j) ? 1.f : 0.f;
-            }
+        for (int e = 0; e < PER; ++e) {
+            const int ic = min(i0 + 2 * e, mc);
+            x[e] = p[(int64_t)ic * v.rs + min(j, ic) * v.cs];
         }
     }
-    if (tid == 0) fail = -1;
-    __syncthreads();
+}
 
-    /*
-     * Blocked right-looking over the 4 x 4 grid of 32 x 32 blocks.  Step p:
-     *   A: wave 0 applies step p-1's update to the diagonal block (p, p) and
-     *      factors it (with its inverse); meanwhile waves 1..3 apply step
-     *      p-1's other trailing updates and assemble row p-1 of the inverse
-     *      (its blocks need only earlier rows and X(p-1, p-1));
-     *   B: panel  L(i, p) := L(i, p) X(p, p)^T  for the blocks below.
-     */
+/* The fetched tile into Lt, identity beyond m */
+__device__ __forceinline__ void leaf_put(float *Lt, const float (&x)[PER], RfpView v, int m, int tid)
+{
+    if (v.rs == 1) {
+        const int i = tid % NB, j0 = tid / NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) Lt[i * LP + j0 + 2 * e] = x[e];
+    } else {
+        const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) Lt[(i0 + 2 * e) * LP + j] = x[e];
+    }
+    if (m < NB) {
+        __syncthreads();
+        for (int idx = tid; idx < NB * NB; idx += 256) {
+            const int i = idx / NB, j = idx % NB;
+            if (i >= m || j >= m) Lt[i * LP + j] = (i == j) ? 1.f : 0.f;
+        }
+    }
+}
+
+/* Load the m x m tile into Lt (all loads in flight before the LDS stores,
+ * the check of info among them); returns false (storing nothing) when an
+ * earlier tile has failed. */
+__device__ __forceinline__ bool leaf_load(float *Lt, const float *base, RfpView v, int m, const int *info, int tid)
+{
+    float x[PER];
+    leaf_fetch(x, base, v, m, tid);
+    if (*info != INT_MAX) return false;
+    leaf_put(Lt, x, v, m, tid);
+    return true;
+}
+
+/*
+ * Blocked right-looking over the 4 x 4 grid of 32 x 32 blocks.  Step p:
+ *   A: wave 0 applies step p-1's update to the diagonal block (p, p) and
+ *      factors it (with its inverse); meanwhile waves 1..3 apply step
+ *      p-1's other trailing updates and assemble row p-1 of the inverse
+ *      (its blocks need only earlier rows and X(p-1, p-1));
+ *   B: panel  L(i, p) := L(i, p) X(p, p)^T  for the blocks below.
+ * Returns the first failing local column or -1 (uniform); rows 0 .. 2 of
+ * the inverse blocks are complete afterwards (leaf_inverse adds the last).
+ */
+__device__ __forceinline__ int leaf_factor(float *Lt, float *Xt, float (*Tt)[BS * 33], int *fail, bool want_inv,
+                                           int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    if (tid == 0) *fail = -1;
+    __syncthreads();
     for (int p = 0; p < NBLK; ++p) {
         if (wave == 0) {
             if (p > 0) {
-                trail_block(L, p, p, p - 1, lane);
+                trail_block(Lt, p, p, p - 1, lane);
                 wave_sync();
             }
-            const int f = factor_diag(L + (p * BS) * LP + p * BS, X + (p * BS) * LP + p * BS, T[0], lane);
-            if (lane == 0 && f >= 0) fail = p * BS + f;
+            const int f = factor_diag(Lt + (p * BS) * LP + p * BS, Xt + (p * BS) * LP + p * BS, Tt[0], lane);
+            if (lane == 0 && f >= 0) *fail = p * BS + f;
         } else if (p > 0) {
             int blk = 0;
             for (int i = p; i < NBLK; ++i)
                 for (int j = p; j <= i; ++j) {
                     if (i == p && j == p) continue;
-                    if (blk++ % 3 == wave - 1) trail_block(L, i, j, p - 1, lane);
+                    if (blk++ % 3 == wave - 1) trail_block(Lt, i, j, p - 1, lane);
                 }
-            if (inv)
-                for (int j = wave - 1; j < p - 1; j += 3) inv_block(L, X, T[wave], p - 1, j, lane);
+            if (want_inv)
+                for (int j = wave - 1; j < p - 1; j += 3) inv_block(Lt, Xt, Tt[wave], p - 1, j, lane);
         }
         __syncthreads();
-        if (fail >= 0) break;
-        const float *Xpp = X + (p * BS) * LP + p * BS;
+        if (*fail >= 0) break;
+        const float *Xpp = Xt + (p * BS) * LP + p * BS;
         for (int i = p + 1 + wave; i < NBLK; i += 4) {
-            float *Lip = L + (i * BS) * LP + p * BS;
+            float *Lip = Lt + (i * BS) * LP + p * BS;
             f32x16_t acc;
             zero(acc);
             mma(acc, Lip, LP, 1, Xpp, 1, LP, lane);   /* B(k, j) = X(j, k) */
@@ -574,42 +597,271 @@ __global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v,
         }
         __syncthreads();
     }
+    return *fail;
+}
 
-    /* write L back (also after a failure: LAPACK leaves a partial factor);
-     * the stores drain while the inverse is assembled */
+/* The factor's lower triangle (m x m) back to the view */
+__device__ __forceinline__ void leaf_store(const float *Lt, float *base, RfpView v, int m, int tid)
+{
     if (v.rs == 1) {
         const int i = tid % NB, j0 = tid / NB;
         if (i < m) {
             float *p = base + v.off + i + (int64_t)j0 * v.cs;
             const int64_t step = 2 * v.cs;
-            for (int j = j0; j <= i; j += 2, p += step) *p = L[i * LP + j];
+            for (int j = j0; j <= i; j += 2, p += step) *p = Lt[i * LP + j];
         }
     } else {
         const int j = tid % NB, i0 = tid / NB;
         float *p = base + v.off + (int64_t)i0 * v.rs + j;
         const int64_t step = 2 * v.rs;
         for (int i = i0; i < m; i += 2, p += step)
-            if (i >= j) *p = L[i * LP + j];
+            if (i >= j) *p = Lt[i * LP + j];
     }
-    if (fail >= 0) {
-        if (tid == 0) atomicMin(info, info_base + fail + 1);
-        return;
-    }
-    if (!inv) return;
+}
 
-    /* the last row of the inverse */
-    if (wave < NBLK - 1) inv_block(L, X, T[wave], NBLK - 1, wave, lane);
+/* The last block row of the inverse, then the dense row-major inverse
+ * (zero above the diagonal and beyond m) to inv; Xt holds the whole inverse
+ * afterwards (its blocks above the diagonal blocks are not written). */
+__device__ __forceinline__ void leaf_inverse(const float *Lt, float *Xt, float (*Tt)[BS * 33], float *inv, int m,
+                                             int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    if (wave < NBLK - 1) inv_block(Lt, Xt, Tt[wave], NBLK - 1, wave, lane);
     __syncthreads();
-    /* dense row-major inverse, zero above the diagonal and beyond m */
 #pragma unroll 1
     for (int e = 0; e < NB * NB / 1024; ++e) {
         const int idx = 4 * (tid + 256 * e);
         const int i = idx / NB, j = idx % NB;
         float w[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w[u] = (i < m && j + u <= i) ? X[i * LP + j + u] : 0.f;
+        for (int u = 0; u < 4; ++u) w[u] = (i < m && j + u <= i) ? Xt[i * LP + j + u] : 0.f;
         *reinterpret_cast<float4 *>(inv + idx) = make_float4(w[0], w[1], w[2], w[3]);
     }
+}
+
+__global__ __launch_bounds__(256) void potrf_leaf_kernel(float *base, RfpView v, int m, int *info, int info_base,
+                                                         float *inv)
+{
+    __shared__ float L[NB * LP];
+    __shared__ float X[NB * LP];
+    __shared__ __attribute__((aligned(16))) float T[4][BS * 33];
+    __shared__ int fail;
+    const int tid = threadIdx.x;
+    if (!leaf_load(L, base, v, m, info, tid)) return;
+    const int f = leaf_factor(L, X, T, &fail, inv != nullptr, tid);
+    /* write L back (also after a failure: LAPACK leaves a partial factor);
+     * the stores drain while the inverse is assembled */
+    leaf_store(L, base, v, m, tid);
+    if (f >= 0) {
+        if (tid == 0) atomicMin(info, info_base + f + 1);
+        return;
+    }
+    if (inv) leaf_inverse(L, X, T, inv, m, tid);
+}
+
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+
+/* acc += A * B^T for one 32 x 32 x 32 block product with A(i, k) at
+ * A[i*LP + k] and B(j, k) at B[j*LP + k] (LDS), both operands rounded to
+ * BF16 (nearest even): two v_mfma_f32_32x32x16_bf16, exact products with
+ * FP32 accumulation (the arithmetic of the BF16 updates).  Lane l supplies
+ * row l % 32 of each operand at k = 16 t + 8 (l / 32) + 0 .. 7; the result
+ * has mma()'s layout. */
+__device__ __forceinline__ void mma_bf16t(f32x16_t &acc, const float *A, const float *B, int lane)
+{
+    const int r = lane & 31, h = lane >> 5;
+    float a[2][8], b[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            a[t][e] = A[r * LP + 16 * t + 8 * h + e];
+            b[t][e] = B[r * LP + 16 * t + 8 * h + e];
+        }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        bf16x8_t av, bv;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            av[e] = (__bf16)a[t][e];
+            bv[e] = (__bf16)b[t][e];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+    }
+}
+
+/*
+ * Two leaves of a 128 < m <= 256 diagonal block in one launch: the
+ * recursion's potrf(m) = potrf(L00); X := C10 L00^-T; C11 -= X X^T;
+ * potrf(C11), which otherwise takes a leaf launch, a panel launch, an
+ * update launch and a second leaf launch (the bottom of spftrf's recursion).
+ * One workgroup: leaf 0 is factored as in potrf_leaf_kernel (its inverse
+ * written to inv); C10 (the m - 128 rows below it) is staged into L's LDS
+ * and multiplied by the inverse still in X (the panel's rows, written to the
+ * view and, in the BF16 modes, to the BF16 shadow view sv); C11 is loaded
+ * into X, updated with the panel's product (BF16-rounded operands when bf16,
+ * like the BF16 updates), and factored as leaf 1 with the roles of the two
+ * LDS arrays exchanged.  info as LAPACK's (first failing column of the
+ * block).
+ */
+__global__ __launch_bounds__(256) void potrf_pair_kernel(float *base, RfpView v, int m, int *info, int info_base,
+                                                         float *inv, unsigned short *sh_hi, unsigned short *sh_lo,
+                                                         RfpView sv, int bf16)
+{
+    __shared__ float L[NB * LP];
+    __shared__ float X[NB * LP];
+    __shared__ __attribute__((aligned(16))) float T[4][BS * 33];
+    __shared__ int fail;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int m2 = m - NB;   /* rows of leaf 1 (1 .. NB) */
+
+    /* leaf 0 */
+    if (!leaf_load(L, base, v, NB, info, tid)) return;
+    const int f0 = leaf_factor(L, X, T, &fail, true, tid);
+    /* the loads of C10 (the m2 x 128 rows below leaf 0; rows at or beyond m2
+     * clamped, zeroed in LDS) are in flight during leaf 0's store and inverse */
+    const RfpView v10 = v.sub(NB, 0);
+    float x10[PER];
+    if (v10.rs == 1) {
+        const float *p = base + v10.off + min(tid % NB, m2 - 1);
+#pragma unroll
+        for (int e = 0; e < PER; ++e) x10[e] = p[(int64_t)(tid / NB + 2 * e) * v10.cs];
+    } else {
+        const float *p = base + v10.off + tid % NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) x10[e] = p[(int64_t)min(tid / NB + 2 * e, m2 - 1) * v10.rs];
+    }
+    leaf_store(L, base, v, NB, tid);
+    if (f0 >= 0) {
+        if (tid == 0) atomicMin(info, info_base + f0 + 1);
+        return;
+    }
+    leaf_inverse(L, X, T, inv, NB, tid);
+    __syncthreads();   /* L's store and every read of it done */
+    if (v10.rs == 1) {
+        const int i = tid % NB, j0 = tid / NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) L[i * LP + j0 + 2 * e] = i < m2 ? x10[e] : 0.f;
+    } else {
+        const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) L[(i0 + 2 * e) * LP + j] = i0 + 2 * e < m2 ? x10[e] : 0.f;
+    }
+    /* C11's loads, in flight during the panel */
+    const RfpView v11 = v.sub(NB, NB);
+    float x11[PER];
+    leaf_fetch(x11, base, v11, m2, tid);
+    __syncthreads();
+
+    /* panel: wave w computes its row slab P(w, jb) = sum_{kb <= jb} C10(w, kb) inv(jb, kb)^T,
+     * jb = 0 .. 3 (ten block products per wave), and overwrites C10(w, .) with it */
+    {
+        f32x16_t acc[NBLK];
+#pragma unroll
+        for (int jb = 0; jb < NBLK; ++jb) {
+            zero(acc[jb]);
+#pragma unroll
+            for (int kb = 0; kb <= jb; ++kb)
+                mma(acc[jb], L + (wave * BS) * LP + kb * BS, LP, 1, X + (jb * BS) * LP + kb * BS, 1, LP, lane);
+        }
+        wave_sync();   /* the slab is the wave's own: its reads are done before it is overwritten */
+#pragma unroll
+        for (int jb = 0; jb < NBLK; ++jb) store(acc[jb], L + (wave * BS) * LP + jb * BS, LP, 1.f, false, lane);
+    }
+    __syncthreads();
+    /* the panel to the view and the BF16 shadow; C11 (its lower triangle,
+     * identity beyond m2) into X, whose inverse of leaf 0 is written out */
+    if (v10.rs == 1) {
+        const int i = tid % NB, j0 = tid / NB;
+        if (i < m2) {
+            float *p = base + v10.off + i + (int64_t)j0 * v10.cs;
+            for (int j = j0; j < NB; j += 2, p += 2 * v10.cs) *p = L[i * LP + j];
+        }
+    } else {
+        const int j = tid % NB, i0 = tid / NB;
+        float *p = base + v10.off + (int64_t)i0 * v10.rs + j;
+        for (int i = i0; i < m2; i += 2, p += 2 * v10.rs) *p = L[i * LP + j];
+    }
+    if (sh_hi) {
+        const RfpView s10 = sv.sub(NB, 0);
+        if (s10.cs == 1 && s10.rs % 8 == 0 && s10.off % 8 == 0) {
+            /* rows k-contiguous and 16-byte aligned: 8 elements per store */
+            constexpr int CH = NB / 8;
+#pragma unroll
+            for (int e = 0; e < NB * CH / 256; ++e) {
+                const int idx = tid + 256 * e, i = idx / CH, c = idx % CH;
+                if (i >= m2) continue;
+                bf16x8_t hv, lv;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float x = L[i * LP + 8 * c + q];
+                    hv[q] = (__bf16)x;   /* nearest even */
+                    lv[q] = (__bf16)(x - (float)hv[q]);
+                }
+                const int64_t so = s10.off + (int64_t)i * s10.rs + 8 * c;
+                *reinterpret_cast<bf16x8_t *>(sh_hi + so) = hv;
+                if (sh_lo) *reinterpret_cast<bf16x8_t *>(sh_lo + so) = lv;
+            }
+        } else {
+            for (int idx = tid; idx < NB * NB; idx += 256) {
+                const int i = s10.cs == 1 ? idx / NB : idx % NB, j = s10.cs == 1 ? idx % NB : idx / NB;
+                if (i >= m2) continue;
+                const float x = L[i * LP + j];
+                const __bf16 hb = (__bf16)x;
+                const int64_t so = s10.off + (int64_t)i * s10.rs + (int64_t)j * s10.cs;
+                sh_hi[so] = __builtin_bit_cast(unsigned short, hb);
+                if (sh_lo) sh_lo[so] = __builtin_bit_cast(unsigned short, (__bf16)(x - (float)hb));
+            }
+        }
+    }
+    leaf_put(X, x11, v11, m2, tid);
+    __syncthreads();
+
+    /* C11 -= P P^T on the 10 blocks on or below the diagonal, block t =
+     * (ib, jb) in row order.  Wave w takes blocks 2w and 2w + 1 whole and half
+     * the k range of block 8 + (w & 1) (waves 0, 1 the first half, stored
+     * first; waves 2, 3 the second, stored after a barrier), ten block
+     * products each. */
+    {
+        auto blk_i = [](int t) { return (int)((0x3333222110ull >> (4 * t)) & 15); };
+        auto blk_j = [](int t) { return (int)((0x3210210100ull >> (4 * t)) & 15); };
+        auto syrk = [&](f32x16_t &c, int t, int k0, auto nk) {
+            const int ib = blk_i(t), jb = blk_j(t);
+            zero(c);
+#pragma unroll
+            for (int q = 0; q < decltype(nk)::value; ++q) {
+                const int kb = k0 + q;
+                const float *Ai = L + (ib * BS) * LP + kb * BS, *Bj = L + (jb * BS) * LP + kb * BS;
+                if (bf16) mma_bf16t(c, Ai, Bj, lane);
+                else      mma(c, Ai, LP, 1, Bj, 1, LP, lane);
+            }
+        };
+        auto put = [&](const f32x16_t &c, int t) {
+            const int ib = blk_i(t), jb = blk_j(t);
+            store(c, X + (ib * BS) * LP + jb * BS, LP, -1.f, true, lane, ib == jb);
+        };
+        f32x16_t c0, c1, ch;
+        const int th = 8 + (wave & 1), k0 = wave < 2 ? 0 : NBLK / 2;
+        syrk(c0, 2 * wave, 0, std::integral_constant<int, NBLK>{});
+        syrk(c1, 2 * wave + 1, 0, std::integral_constant<int, NBLK>{});
+        syrk(ch, th, k0, std::integral_constant<int, NBLK / 2>{});
+        put(c0, 2 * wave);
+        put(c1, 2 * wave + 1);
+        if (wave < 2) put(ch, th);
+        __syncthreads();
+        if (wave >= 2) put(ch, th);
+    }
+    __syncthreads();
+
+    /* leaf 1: factor X in place, its inverse in L */
+    const int f1 = leaf_factor(X, L, T, &fail, true, tid);
+    leaf_store(X, base, v.sub(NB, NB), m2, tid);
+    if (f1 >= 0) {
+        if (tid == 0) atomicMin(info, info_base + NB + f1 + 1);
+        return;
+    }
+    leaf_inverse(X, L, T, inv + NB * NB, m2, tid);
 }
 
 }  // namespace leaf32
@@ -739,6 +991,18 @@ static hipError_t trtri_tiles(const T *base, RfpView v, int64_t m, T *inv, hipSt
 
 hipError_t launch_potrf_tile_f32(float *base, RfpView v, int m, int *info, int info_base, float *inv, hipStream_t s)
 { return potrf_tile<float>(base, v, m, info, info_base, inv, s); }
+
+int potrf_pair_max() { return 2 * LeafSize<float>::value; }
+
+hipError_t launch_potrf_pair_f32(float *base, RfpView v, int m, int *info, int info_base, float *inv,
+                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView sv, bool bf16, hipStream_t s)
+{
+    constexpr int NB = LeafSize<float>::value;
+    if (m <= NB || m > 2 * NB || !inv) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(leaf32::potrf_pair_kernel, dim3(1), dim3(256), 0, s, base, v, m, info, info_base, inv, sh_hi,
+                       sh_lo, sv, bf16 ? 1 : 0);
+    return hipGetLastError();
+}
 hipError_t launch_potrf_tile_f64(double *base, RfpView v, int m, int *info, int info_base, double *inv, hipStream_t s)
 { return potrf_tile<double>(base, v, m, info, info_base, inv, s); }
 hipError_t launch_trtri_tiles_f32(const float *base, RfpView v, int64_t m, float *inv, hipStream_t s)

@@ -722,6 +722,14 @@ struct Factor {
             return timed(h, CURFP_KERNEL_POTRF_TILE, (double)m * m * m / 3.0, 0.0, [&] {
                 return Prec<T>::potrf_tile(a, V, (int)m, h->d_info, (int)goff, inv + slot * leaf * leaf, h->stream);
             });
+        if (!Prec<T>::f64 && m <= potrf_pair_max() && math != RFP_MATH_BF16X3)
+            /* FP32 / BF16: the two leaves, the panel and the update between
+             * them in one workgroup (four launches otherwise) */
+            return timed(h, CURFP_KERNEL_POTRF_TILE, (double)m * m * m / 3.0, 0.0, [&] {
+                return launch_potrf_pair_f32((float *)a, V, (int)m, h->d_info, (int)goff,
+                                             (float *)(inv + slot * leaf * leaf), sh_hi, sh_lo, S,
+                                             math == RFP_MATH_BF16, h->stream);
+            });
         const int64_t m1 = leaf_split(m, leaf);
         return potrf2(V, S, V.sub(m1, 0), S.sub(m1, 0), V.sub(m1, m1), S.sub(m1, m1), m1, m - m1, goff, slot,
                       slot + m1 / leaf);

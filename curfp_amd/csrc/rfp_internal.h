@@ -258,6 +258,17 @@ hipError_t launch_potrf_tile_f64(double *base, RfpView v, int m, int *info, int 
                                  double *inv, hipStream_t s);
 
 /*
+ * FP32 Cholesky of a diagonal block of leaf < m <= 2 leaf rows in one launch:
+ * the two leaves, the panel below the first (also written to the BF16
+ * shadow view sv when sh_hi != nullptr) and the update of the second (its
+ * operands rounded to BF16 when bf16).  The two leaf inverses go to inv and
+ * inv + leaf*leaf; info as launch_potrf_tile_f32 for the whole block.
+ */
+int potrf_pair_max();
+hipError_t launch_potrf_pair_f32(float *base, RfpView v, int m, int *info, int info_base, float *inv,
+                                 unsigned short *sh_hi, unsigned short *sh_lo, RfpView sv, bool bf16, hipStream_t s);
+
+/*
  * Batched inverse of lower-triangular diagonal tiles: tile t covers rows
  * [t*leaf, min(m, (t+1)*leaf)) of the lower-triangular view v (m x m);
  * inv(tile t) goes to inv + t*leaf*leaf (row-major, zero upper part).

@@ -18,8 +18,10 @@
  *       workgroup replayed
  *   panel_replay chain n transr uplo bf16 [all]
  *       every panel launch of spftrf at order n (Factor::potrf / potrf2 /
- *       trsm_rt of rfp_drivers.hip), for the FP32 (bf16 = 0) or BF16 modes
- *       (bf16 = 1: shadow writes, panels of at most BF16_MAX_ROWS rows);
+ *       trsm_rt of rfp_drivers.hip), for the FP32 (bf16 = 0), BF16 (1) or
+ *       split-BF16 (3) modes (BF16: shadow writes, panels of at most
+ *       BF16_MAX_ROWS rows; FP32 and BF16: no panel launch inside a block of
+ *       up to two leaves, which is one pair launch);
  *       workgroups 0, 1 and the last two of each launch (every address of a
  *       workgroup is its first row r0 = 32 * blockIdx.x times the row stride
  *       plus a pattern that depends on blockIdx.x only through the row count
@@ -280,6 +282,7 @@ struct Chain {
     Replay &rp;
     int64_t leaf;
     bool bf16;
+    bool pair;   /* FP32 / BF16: blocks of up to two leaves are one pair launch */
 
     void trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
@@ -302,7 +305,7 @@ struct Chain {
     }
     void potrf(RfpView V, RfpView S, int64_t m, int64_t slot)
     {
-        if (m <= leaf) return;
+        if (m <= leaf || (pair && m <= 2 * leaf)) return;
         const int64_t m1 = leaf_split(m, leaf);
         potrf2(V, S, V.sub(m1, 0), S.sub(m1, 0), V.sub(m1, m1), S.sub(m1, m1), m1, m - m1, slot, slot + m1 / leaf);
     }
@@ -329,7 +332,8 @@ int main(int argc, char **argv)
     } else if (!std::strcmp(argv[1], "chain") && argc >= 6) {
         const int64_t n = std::atoll(argv[2]);
         const bool tn = argv[3][0] == 'N', lower = argv[4][0] == 'L';
-        rp.bf16 = std::atoi(argv[5]) != 0;
+        const int mode = std::atoi(argv[5]);   /* 0 FP32, 1 BF16, 3 split BF16 */
+        rp.bf16 = mode != 0;
         rp.all_wg = argc > 6 && !std::strcmp(argv[6], "all");
         const RfpGeom g = rfp_geom(n, tn, lower);
         const ShadowGeom sg = shadow_geom(g);
@@ -339,7 +343,7 @@ int main(int argc, char **argv)
         rp.inv = Bounds{"inverses", 0, (s1 + s2) * leaf * leaf};
         rp.shadow = Bounds{"shadow", 0, rp.bf16 ? sg.elems : 0};
         rp.where = "chain n=" + std::to_string(n) + " " + argv[3] + argv[4] + (rp.bf16 ? " bf16" : " fp32");
-        Chain ch{rp, leaf, rp.bf16};
+        Chain ch{rp, leaf, rp.bf16, mode != 3};
         ch.potrf2(g.v11, sg.s11, g.v21, sg.s21, g.v22, sg.s22, g.n1, g.n2, 0, s1);
     } else if (!std::strcmp(argv[1], "chain64") && argc >= 5) {
         const int64_t n = std::atoll(argv[2]);

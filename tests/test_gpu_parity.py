@@ -204,6 +204,40 @@ def test_pftrf_info_exact(curfp, golden):
         curfp.spftrf(dev(-np.ones(3, np.float32)), check=True)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("t,u", VARIANTS)
+def test_spftrf_two_leaf_blocks(curfp, mode, t, u):
+    """Orders whose recursion ends in diagonal blocks of 129 .. 256 rows (one
+    launch for both leaves, the panel and the update between them in FP32 and
+    BF16 mode; the four-launch recursion in split-BF16): backward error at the
+    mode's precision, and exact LAPACK info for a failing pivot in the first
+    leaf, on the leaf boundary, in the second leaf and in the second half.
+    A diagonal entry set to -100 in an SPD matrix with eigenvalues >= 1 makes
+    that column's pivot the first non-positive one."""
+    rng = np.random.default_rng(11)
+    u32 = np.finfo(np.float32).eps
+    for n in [129, 200, 256, 257, 300, 511, 512, 700]:
+        A = rng.standard_normal((n, 64)) / 8.0
+        S = (A @ A.T + np.eye(n)).astype(np.float32)
+        S64 = S.astype(np.float64)
+        arf = dev(O.trttf(S, t, u))
+        with curfp.math_mode(mode):
+            assert curfp.spftrf(arf, transr=t, uplo=u, check=False) == 0, n
+        L = O.rfp_lower_factor(arf.cpu().numpy().astype(np.float64), n, t, u)
+        err = np.linalg.norm(L @ L.T - S64) / np.linalg.norm(S64)
+        assert err <= (2.0 ** -7 if mode == "bf16" else 8 * math.sqrt(n) * u32), (n, err)
+    for n, cols in [(300, [1, 100, 128, 129, 140, 150, 151, 250, 300]),
+                    (512, [129, 200, 256, 257, 385, 512]), (384, [129, 192, 193, 320])]:
+        A = rng.standard_normal((n, 64)) / 8.0
+        S = (A @ A.T + np.eye(n)).astype(np.float32)
+        for p in cols:
+            Sp = S.copy()
+            Sp[p - 1, p - 1] = -100.0
+            with curfp.math_mode(mode):
+                info = curfp.spftrf(dev(O.trttf(Sp, t, u)), transr=t, uplo=u, check=False)
+            assert info == p, (n, p, info)
+
+
 # ----------------------------------------------------------------- larger sizes vs oracle
 
 def _spd_problem(n, k, nrhs, seed, dtype=np.float32, lam=1.0):

@@ -6,7 +6,8 @@
   m = 512: 1024 workgroups) — every workgroup's every access lies inside the
   array of 512 + 32768 rows, and a buffer one row short is caught.
 * Every panel launch of spftrf at n = 65536 and 65535 in the four layouts,
-  FP32 and BF16 (shadow writes; panels of at most 16384 rows), and at small
+  FP32, BF16 and split BF16 (shadow writes; panels of at most 16384 rows;
+  no panel launch inside a two-leaf block except in split BF16), and at small
   ragged sizes with every workgroup replayed: reads and writes stay inside
   the RFP array, the leaf-inverse slots and the BF16 shadow, X elements
   inside the launch's mb x m block and L reads inside the strict lower
@@ -54,7 +55,7 @@ def test_probe_replay_catches_a_short_buffer(replay):
 
 @pytest.mark.parametrize("n", [65536, 65535])
 @pytest.mark.parametrize("t,u", [("N", "L"), ("N", "U"), ("T", "L"), ("T", "U")])
-@pytest.mark.parametrize("bf16", [0, 1])
+@pytest.mark.parametrize("bf16", [0, 1, 3])
 def test_spftrf_panels_in_bounds(replay, n, t, u, bf16):
     r = replay("chain", n, t, u, bf16)
     assert r.returncode == 0, r.stdout
@@ -63,7 +64,7 @@ def test_spftrf_panels_in_bounds(replay, n, t, u, bf16):
 
 @pytest.mark.parametrize("n", [129, 640, 1025, 4097, 20001])
 @pytest.mark.parametrize("t,u", [("N", "U"), ("T", "L")])
-@pytest.mark.parametrize("bf16", [0, 1])
+@pytest.mark.parametrize("bf16", [0, 1, 3])
 def test_spftrf_panels_every_workgroup(replay, n, t, u, bf16):
     r = replay("chain", n, t, u, bf16, "all")
     assert r.returncode == 0, r.stdout
