@@ -205,6 +205,32 @@ def test_pftrf_info_exact(curfp, golden):
 
 
 @pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
+@pytest.mark.parametrize("n,t,u", [(16385, "N", "L"), (17000, "T", "U"), (20001, "T", "L"), (24576, "N", "U")])
+def test_spftrf_tall_panels(curfp, mode, n, t, u):
+    """Orders whose panel solves include panels of 8192 .. 12288 rows (one
+    panel launch per 512 columns) and, at 24576, the two 12288-row halves of
+    the top-level solve: ||L L^T - C||_F / ||C||_F formed in FP64 on the GPU,
+    at the mode's precision."""
+    g = torch.Generator(device="cuda").manual_seed(n)
+    A = torch.randn(n, 256, device="cuda", generator=g) / 16.0
+    C = torch.empty(n * (n + 1) // 2, device="cuda")
+    curfp.ssfrk(A, C, transr=t, uplo=u)
+    curfp.add_to_diagonal(C, 1.0, transr=t, uplo=u)
+    T = curfp.stfttr(C, transr=t, uplo=u).double()
+    S = T + T.T - torch.diag(torch.diagonal(T))
+    del T
+    with curfp.math_mode(mode):
+        assert curfp.spftrf(C, transr=t, uplo=u) == 0
+    F = curfp.stfttr(C, transr=t, uplo=u).double()
+    L = F if u == "L" else F.T.contiguous()
+    del F
+    err = (torch.linalg.norm(L @ L.T - S) / torch.linalg.norm(S)).item()
+    u32 = np.finfo(np.float32).eps
+    bound = 2.0 ** -7 if mode == "bf16" else (1e-4 if mode == "bf16x3" else 4 * math.sqrt(n) * u32)
+    assert err <= bound, (n, mode, err)
+
+
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
 @pytest.mark.parametrize("t,u", VARIANTS)
 def test_spftrf_two_leaf_blocks(curfp, mode, t, u):
     """Orders whose recursion ends in diagonal blocks of 129 .. 256 rows (one
