@@ -23,7 +23,7 @@ INCLUDE = os.path.join(ROOT, "include")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 BUILD = os.path.join(HERE, "_objs")
 
-HIP_SOURCES = ["gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "trsm_panel64.hip", "lansf.hip", "pfcon.hip", "rfp_drivers.hip"]
+HIP_SOURCES = ["blas.hip", "gemm.hip", "gemm_bf16.hip", "gemm_bf16_dma.hip", "potrf.hip", "pack.hip", "refine.hip", "trsv_block.hip", "trsm_panel.hip", "trsm_panel64.hip", "lansf.hip", "pfcon.hip", "rfp_drivers.hip"]
 LIB = os.path.join(HERE, "libcurfp_amd.so")
 
 HIPCC_FLAGS = [
@@ -75,7 +75,8 @@ def build(verbose: bool = True) -> None:
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         objs = list(ex.map(_compile, HIP_SOURCES))
     tmp = LIB + ".tmp"
-    _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([os.path.join(ROCM, "bin", "hipcc"), "--offload-arch=gfx950", "-shared", "-fPIC", *objs,
+          "-L" + os.path.join(ROCM, "lib"), "-lhipblaslt", "-lrocblas", "-o", tmp])
     os.replace(tmp, LIB)
     if verbose:
         print(f"[curfp_amd] built {LIB}")

@@ -158,7 +158,21 @@ struct curfpContext {
     std::vector<RfpProfRecord> prof_pending;
     std::vector<hipEvent_t>    prof_pool;
     RfpKernelStat   prof_stats[CURFP_KERNEL_CLASSES];
+    /* vendor-library GEMM state (blas.hip), created on first use */
+    void           *blas;
+    bool            blas_off;  /* CURFP_GEMM_NATIVE set: every GEMM on the hand-written kernels */
 };
+
+/*
+ * Plain GEMM updates on the vendor libraries (blas.hip).  blas_route: 0 when
+ * the hand-written kernels take g (lower-triangular outputs, in-place solves,
+ * shadow epilogues, operands converted while staged, mixed precision, small
+ * or non-unit-stride problems), else the library route for blas_gemm.
+ */
+int blas_route(const curfpHandle_t h, const struct GemmArgs &g, int math, bool f64);
+hipError_t blas_gemm(curfpHandle_t h, const struct GemmArgs &g, int math, int route);
+void blas_destroy(void *state);
+bool blas_native_only();
 
 /* Returns a device workspace of at least `bytes` bytes owned by the handle. */
 hipError_t rfp_workspace(curfpHandle_t h, size_t bytes, void **out);

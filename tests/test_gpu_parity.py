@@ -230,6 +230,56 @@ def test_spftrf_tall_panels(curfp, mode, n, t, u):
     assert err <= bound, (n, mode, err)
 
 
+@pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3", "fp64"])
+def test_gemm_routes_agree(curfp, mode, monkeypatch):
+    """The large plain GEMM updates go to hipBLASLt / rocBLAS; a handle created
+    with CURFP_GEMM_NATIVE=1 keeps every GEMM on the hand-written kernels.
+    Both factorisations of the same matrix (n = 20001, with updates of up to
+    10001 x 10000 x 10000 and SYRKs split into library rectangles) meet the
+    mode's backward-error bound and agree to within the condition number
+    times it; each route repeats bit-identically."""
+    n, t, u = 20001, "T", "U"
+    f64 = mode == "fp64"
+    dt = torch.float64 if f64 else torch.float32
+    g = torch.Generator(device="cuda").manual_seed(5)
+    A = (torch.randn(n, 256, device="cuda", generator=g) / 16.0).to(dt)
+    C0 = torch.empty(n * (n + 1) // 2, device="cuda", dtype=dt)
+    (curfp.dsfrk if f64 else curfp.ssfrk)(A, C0, transr=t, uplo=u)
+    curfp.add_to_diagonal(C0, 1.0, transr=t, uplo=u)
+    T = (curfp.dstfttr if f64 else curfp.stfttr)(C0, transr=t, uplo=u).double()
+    S = T + T.T - torch.diag(torch.diagonal(T))
+    del T
+    math_code = {"fp32": curfp.MATH_FP32, "bf16": curfp.MATH_BF16, "bf16x3": curfp.MATH_BF16X3, "fp64": curfp.MATH_FP32}
+    factors = {}
+    for route in ("library", "native"):
+        if route == "native":
+            monkeypatch.setenv("CURFP_GEMM_NATIVE", "1")
+        else:
+            monkeypatch.delenv("CURFP_GEMM_NATIVE", raising=False)
+        with curfp.Handle() as h:
+            h.set_math_mode(math_code[mode])
+            out = []
+            for _ in range(2):
+                C = C0.clone()
+                fn = curfp.dpftrf_raw if f64 else curfp.spftrf_raw
+                assert fn(h, curfp._op(t), curfp._fill(u), n, C) == 0
+                torch.cuda.synchronize()
+                out.append(C)
+            assert torch.equal(out[0], out[1]), route
+            factors[route] = out[0]
+    u_ = np.finfo(np.float64 if f64 else np.float32).eps
+    bound = 2.0 ** -7 if mode == "bf16" else (1e-4 if mode == "bf16x3" else 4 * math.sqrt(n) * u_)
+    Ls = {}
+    for route, C in factors.items():
+        F = (curfp.dstfttr if f64 else curfp.stfttr)(C, transr=t, uplo=u).double()
+        L = F.T.contiguous()
+        err = (torch.linalg.norm(L @ L.T - S) / torch.linalg.norm(S)).item()
+        assert err <= bound, (route, mode, err)
+        Ls[route] = L
+    diff = (torch.linalg.norm(Ls["library"] - Ls["native"]) / torch.linalg.norm(Ls["native"])).item()
+    assert diff <= 200 * bound, (mode, diff)   # condition number of S ~ 80
+
+
 @pytest.mark.parametrize("mode", ["fp32", "bf16", "bf16x3"])
 @pytest.mark.parametrize("t,u", VARIANTS)
 def test_spftrf_two_leaf_blocks(curfp, mode, t, u):
