@@ -112,11 +112,17 @@ def run_bf16_ir_step(curfp, refine, A, C, B0, lam, tol, max_iter, ev=None):
 
 
 KERNEL_INFO = {
-    "gemm_f32": ("view_gemm_kernel<float> (FP32 MFMA v_mfma_f32_32x32x2f32)", "fp32"),
-    "gemm_bf16": ("gemm_bf16_dma_kernel<256|128> (+ gemm_bf16_rows_kernel<64> for small updates; BF16 MFMA "
-                  "v_mfma_f32_32x32x16_bf16, FP32 accumulate, direct global->LDS loads)", "bf16"),
-    "gemm_f64": ("view_gemm_kernel<double> (FP64 MFMA v_mfma_f64_16x16x4f64)", "fp64"),
-    "potrf_tile": ("potrf_leaf_kernel (128-row Cholesky leaf + its inverse, FP32 MFMA)", "fp32"),
+    "gemm_f32": ("FP32 GEMM updates: hipBLASLt FP32 (the plain updates with M, N, K >= 512 and the rectangles "
+                 "of split SYRKs) + view_gemm_kernel<float> (SYRK triangles, small and in-place updates; "
+                 "FP32 MFMA v_mfma_f32_32x32x2f32)", "fp32"),
+    "gemm_bf16": ("BF16 GEMM updates: hipBLASLt BF16->FP32 (the plain updates with M, N, K >= 512 and the "
+                  "rectangles of split SYRKs) + gemm_bf16_dma_kernel<256|128> / gemm_bf16_rows_kernel<64> "
+                  "(SYRK triangles and small updates; v_mfma_f32_32x32x16_bf16, direct global->LDS loads)",
+                  "bf16"),
+    "gemm_f64": ("FP64 GEMM updates: rocBLAS DGEMM (M, N, K >= 256) + view_gemm_kernel<double> (SYRK "
+                 "triangles, small and in-place updates; FP64 MFMA v_mfma_f64_16x16x4f64)", "fp64"),
+    "potrf_tile": ("potrf_pair_kernel / potrf_leaf_kernel (128-row Cholesky leaves with their inverses, two "
+                   "per launch for 129..256-row blocks, FP32 MFMA)", "fp32"),
     "trsv": ("sweep_solve_kernel (whole-factor substitution over 128-row blocks, FP32 MFMA)", "fp32"),
     "trsm": ("trsm_panel_kernel (spftrf X := X L^-T over <= 4 leaves per launch, FP32 MFMA)", "fp32"),
 }
