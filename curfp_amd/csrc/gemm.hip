@@ -77,6 +77,8 @@ struct KParams {
     int64_t c_zstride;  /* element stride between the partial C of two slices */
     unsigned short *sh_hi, *sh_lo;  /* optional BF16 copy of C (hi, lo parts) */
     int64_t sh_off, sh_rs, sh_cs;
+    int batch;                      /* diagonal blocks along blockIdx.z (GemmArgs::batch) */
+    int64_t a_bs, b_bs, c_bs, m_last;
 };
 
 __device__ __forceinline__ unsigned short bf16_rne(float x)
@@ -196,9 +198,18 @@ __global__ __launch_bounds__(256) void view_gemm_kernel(KParams p)
     const int wave = tid >> 6;
     const int wm = wave % WAVES_M, wn = wave / WAVES_M;
 
+    if (p.batch > 1) {
+        /* diagonal block bz of a batched lower-triangular launch */
+        const int64_t bz = blockIdx.z;
+        p.a_off += bz * p.a_bs;
+        p.b_off += bz * p.b_bs;
+        p.c_off += bz * p.c_bs;
+        if (bz == p.batch - 1) p.M = p.N = p.m_last;
+    }
     int64_t tm, tn;
     tile_coords(p, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
     const int64_t m0 = tm * BM, n0 = tn * BN;
+    if (m0 >= p.M || n0 >= p.N) return;   /* past a smaller last block (whole workgroup) */
 
     /* split-K: slice z covers k in [k_lo, k_hi) and writes its own partial C */
     const int64_t z = blockIdx.y;
@@ -325,6 +336,10 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
     p.c_zstride = 0;
     p.sh_hi = g.c_sh_hi; p.sh_lo = g.c_sh_lo;
     p.sh_off = g.vcs.off; p.sh_rs = g.vcs.rs; p.sh_cs = g.vcs.cs;
+    p.batch = g.batch > 1 ? g.batch : 1;
+    p.a_bs = g.a_bs; p.b_bs = g.b_bs; p.c_bs = g.c_bs; p.m_last = g.m_last;
+    if (p.batch > 1 && (!g.lower || g.splits > 1 || g.inplace || g.c_sh_hi || g.m_last > g.M || g.m_last < 1))
+        return hipErrorInvalidValue;
     if (g.lower) {
         if (BM != BN || g.M != g.N) return hipErrorInvalidValue;
         p.ntiles = p.tiles_m * (p.tiles_m + 1) / 2;
@@ -354,7 +369,7 @@ static hipError_t launch_cfg(const GemmArgs &g, hipStream_t s)
         p.alpha = 1.0; p.beta = 0.0;
         p.sh_hi = p.sh_lo = nullptr;
     }
-    dim3 grid((unsigned)p.ntiles, (unsigned)splits), block(256);
+    dim3 grid((unsigned)p.ntiles, (unsigned)splits, (unsigned)p.batch), block(256);
     if (akc && bkc)  hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, true, true>), grid, block, 0, s, p);
     else if (akc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, true, false>), grid, block, 0, s, p);
     else if (bkc)    hipLaunchKernelGGL((view_gemm_kernel<T, AS, BS, BM, BN, WAVES_M, BK, false, true>), grid, block, 0, s, p);
@@ -376,7 +391,7 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s);
 static int64_t tiles128(const GemmArgs &g)
 {
     const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
-    return g.lower ? tm * (tm + 1) / 2 : tm * tn;
+    return (g.lower ? tm * (tm + 1) / 2 : tm * tn) * (g.batch > 1 ? g.batch : 1);
 }
 
 /*

@@ -271,6 +271,8 @@ struct RowParams {
     float alpha, beta;
     int64_t tiles_m, tiles_n, ntiles;
     int lower;
+    int batch;                      /* diagonal blocks along blockIdx.y (GemmArgs::batch) */
+    int64_t a_bs, b_bs, c_bs, m_last;
 };
 
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
@@ -328,6 +330,14 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
     const int wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
 
+    if (p.batch > 1) {
+        /* diagonal block bz of a batched lower-triangular launch */
+        const int64_t bz = blockIdx.y;
+        p.a_off += bz * p.a_bs;
+        p.b_off += bz * p.b_bs;
+        p.c_off += bz * p.c_bs;
+        if (bz == p.batch - 1) p.M = p.N = p.m_last;
+    }
     int64_t tm, tn;
     {
         BKParams q;
@@ -335,6 +345,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_rows_kernel(RowParams p)
         tile_coords(q, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
     }
     const int64_t m0 = tm * BT, n0 = tn * BT;
+    if (m0 >= p.M || n0 >= p.N) return;   /* past a smaller last block (whole workgroup) */
 
     f32x16 acc[FM][FM];
 #pragma unroll
@@ -446,6 +457,9 @@ hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
     p.M = g.M; p.N = g.N; p.K = g.K;
     p.alpha = (float)g.alpha; p.beta = (float)g.beta;
     p.lower = g.lower ? 1 : 0;
+    p.batch = g.batch > 1 ? g.batch : 1;
+    p.a_bs = g.a_bs; p.b_bs = g.b_bs; p.c_bs = g.c_bs; p.m_last = g.m_last;
+    if (p.batch > 1 && (!g.lower || g.m_last > g.M || g.m_last < 1)) return hipErrorInvalidValue;
     p.tiles_m = (g.M + BT - 1) / BT;
     p.tiles_n = (g.N + BT - 1) / BT;
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
@@ -453,8 +467,9 @@ hipError_t launch_rows(const GemmArgs &g, hipStream_t s)
     if (p.ntiles > 0x7fffffffLL) return hipErrorInvalidValue;
     /* row view read and written (beta != 0): transposed accumulators, 16-byte runs along rows */
     const bool swap = g.vc.cs == 1 && g.vc.rs != 1 && g.beta != 0.0;
-    if (swap) hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, true>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
-    else      hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, false>), dim3((unsigned)p.ntiles), dim3(256), 0, s, p);
+    const dim3 grid((unsigned)p.ntiles, (unsigned)p.batch);
+    if (swap) hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, true>), grid, dim3(256), 0, s, p);
+    else      hipLaunchKernelGGL((gemm_bf16_rows_kernel<BT, X3, BK, false>), grid, dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
@@ -546,7 +561,7 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
          * workgroups, a quarter of the work and half the k-slices each).
          * Both thresholds measured on the spftrf chain at n = 65536. */
         const int64_t tm = (g.M + 127) / 128, tn = (g.N + 127) / 128;
-        const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) < 256;
+        const bool small = (g.lower ? tm * (tm + 1) / 2 : tm * tn) * (g.batch > 1 ? g.batch : 1) < 256;
         const bool big = g.M >= 4096 && g.N >= 4096;
         if (x3) return small ? launch_rows<64, true, 64>(g, s) : launch_rows<128, true, 32>(g, s);
         if (small) return launch_rows<64, false, 64>(g, s);
@@ -554,6 +569,7 @@ hipError_t launch_gemm_bf16(const GemmArgs &g, RfpMath math, hipStream_t s)
         if (e != hipErrorNotSupported) return e;
         return big ? launch_rows<256, false, 64>(g, s) : launch_rows<128, false, 32>(g, s);
     }
+    if (g.batch > 1) return hipErrorInvalidValue;   /* batched launches take the row-block kernels */
     BKParams p;
     p.a = g.a; p.a_off = g.va.off; p.a_rs = g.va.rs; p.a_cs = g.va.cs;
     p.b = g.b; p.b_off = g.vb.off; p.b_rs = g.vb.rs; p.b_cs = g.vb.cs;

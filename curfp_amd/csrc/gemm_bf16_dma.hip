@@ -57,6 +57,8 @@ struct DmaParams {
     float alpha, beta;
     int64_t tiles_m, tiles_n, ntiles;
     int lower;
+    int batch;                      /* diagonal blocks along blockIdx.y (GemmArgs::batch) */
+    int64_t a_bs, b_bs, c_bs, m_last;
 };
 
 __device__ __forceinline__ int64_t xcd_remap(int64_t pid, int64_t total)
@@ -186,12 +188,21 @@ __global__ __launch_bounds__(256) void gemm_bf16_dma_kernel(DmaParams p)
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave & 1, wn = wave >> 1;
 
+    if (p.batch > 1) {
+        /* diagonal block bz of a batched lower-triangular launch */
+        const int64_t bz = blockIdx.y;
+        p.a += bz * p.a_bs;
+        p.b += bz * p.b_bs;
+        p.c_off += bz * p.c_bs;
+        if (bz == p.batch - 1) p.M = p.N = p.m_last;
+    }
     int64_t tm, tn;
     tile_coords(p, xcd_remap(blockIdx.x, p.ntiles), tm, tn);
     /* the tile is workgroup-uniform: keep it (and the buffer descriptors
      * built from it) in scalar registers */
     const int64_t m0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tm) * BT;
     const int64_t n0 = (int64_t)__builtin_amdgcn_readfirstlane((int)tn) * BT;
+    if (m0 >= p.M || n0 >= p.N) return;   /* past a smaller last block (whole workgroup) */
     const int64_t ra = (p.M - m0) < BT ? (p.M - m0) : BT;
     const int64_t rb = (p.N - n0) < BT ? (p.N - n0) : BT;
     const int lda2 = (int)(p.a_ld * 2), ldb2 = (int)(p.b_ld * 2);
@@ -344,6 +355,11 @@ hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s)
     p.M = g.M; p.N = g.N; p.K = g.K;
     p.alpha = (float)g.alpha; p.beta = (float)g.beta;
     p.lower = g.lower ? 1 : 0;
+    p.batch = g.batch > 1 ? g.batch : 1;
+    p.a_bs = g.a_bs; p.b_bs = g.b_bs; p.c_bs = g.c_bs; p.m_last = g.m_last;
+    if (p.batch > 1 && (!g.lower || g.m_last > g.M || g.m_last < 1)) return hipErrorInvalidValue;
+    /* a later block's row offsets must stay within the 32-bit descriptors too */
+    if (p.batch > 1 && ((p.a_bs % 8) || (p.b_bs % 8))) return hipErrorNotSupported;
     p.tiles_m = (g.M + bt - 1) / bt;
     p.tiles_n = (g.N + bt - 1) / bt;
     p.ntiles = g.lower ? p.tiles_m * (p.tiles_m + 1) / 2 : p.tiles_m * p.tiles_n;
@@ -355,7 +371,7 @@ hipError_t launch_gemm_bf16_dma(const GemmArgs &g, int bt, hipStream_t s)
      * plain layout's 4-byte stores already cover whole 128-byte lines of a
      * row per instruction and measured faster (ssfrk). */
     const bool swap = g.vc.cs == 1 && g.vc.rs != 1 && g.beta != 0.0;
-    const dim3 grid((unsigned)p.ntiles);
+    const dim3 grid((unsigned)p.ntiles, (unsigned)p.batch);
     if (bt == 256) {
         if (swap) hipLaunchKernelGGL((gemm_bf16_dma_kernel<256, true>), grid, dim3(256), 0, s, p);
         else      hipLaunchKernelGGL((gemm_bf16_dma_kernel<256, false>), grid, dim3(256), 0, s, p);

@@ -338,12 +338,17 @@ template <> struct Prec<double> {
     static constexpr bool f64 = true;
 };
 
-/* Lower-triangular outputs of at least this order are split into two
- * triangles and the rectangle below them (the rectangle to the vendor
- * library); smaller ones stay whole on the hand-written SYRK tiles.
- * Measured on the n = 65536 chains: 8192 for BF16 operands (16384: +1.8 ms
- * per BF16+IR step), 16384 for FP32 and FP64 (8192: +7 / +6 ms). */
-static int64_t lower_split_min(const GemmArgs &g) { return (g.bf16_rows || g.a_bf16) ? 8192 : 16384; }
+/* Lower-triangular outputs of at least this order are cut into diagonal
+ * blocks of order below it (for the hand-written SYRK tiles, one batched
+ * launch) and the row panels left of them (for the vendor library).
+ * Measured on the n = 65536 chains (FP64 n = 32768): BF16 operands 8192
+ * (4096: +5.6 ms per BF16+IR step, 16384: +1.6 ms), FP32 4096 (8192:
+ * +3.7 ms, 16384: +12 ms), FP64 8192 (4096: +5.7 ms, 16384: +3.2 ms). */
+static int64_t lower_split_min(const GemmArgs &g, bool f64)
+{
+    if (g.bf16_rows || g.a_bf16) return 8192;
+    return f64 ? 8192 : 4096;
+}
 
 /* Algorithmic flops of one view GEMM: 2*M*N*K, or M*(M+1)*K for a
  * lower-triangular (SYRK) output. */
@@ -377,27 +382,44 @@ static hipError_t run_gemm(curfpHandle_t h, const GemmArgs &g, RfpMath m)
     if (g.M <= 0 || g.N <= 0) return hipSuccess;
     const int cls = Prec<T>::f64 ? CURFP_KERNEL_GEMM_F64
                                  : ((m == RFP_MATH_F32 && !g.a_bf16) ? CURFP_KERNEL_GEMM_F32 : CURFP_KERNEL_GEMM_BF16);
-    if (g.lower && g.M >= lower_split_min(g)) {
-        /* a large lower-triangular output: the rectangle below its two
-         * half-size triangles as a plain GEMM (the vendor library where it
-         * takes it), the triangles recursively */
-        GemmArgs r = g;
-        r.lower = false;
-        r.splits = 1;
-        r.M = g.M - g.M / 2;
-        r.N = g.M / 2;
-        r.va = g.va.sub(r.N, 0);
-        r.vc = g.vc.sub(r.N, 0);
-        if (blas_route(h, r, m, Prec<T>::f64)) {
-            GemmArgs t1 = g, t2 = g;
-            t1.M = t1.N = r.N;
-            t2.M = t2.N = r.M;
-            t2.va = g.va.sub(r.N, 0);
-            t2.vb = g.vb.sub(r.N, 0);
-            t2.vc = g.vc.sub(r.N, r.N);
-            RFP_TRY(run_gemm<T>(h, t1, m));
-            RFP_TRY(run_gemm<T>(h, r, m));
-            return run_gemm<T>(h, t2, m);
+    if (g.lower && g.batch <= 1 && g.splits <= 1 && !g.inplace && !g.c_sh_hi && g.M >= lower_split_min(g, Prec<T>::f64) &&
+        (Prec<T>::f64 ? !g.a_f32 : (g.bf16_rows || (m == RFP_MATH_F32 && !g.a_bf16)))) {
+        /* A large lower-triangular output: nb diagonal blocks of order S (the
+         * last one m_last) and the row panels left of them.  The panels are
+         * plain GEMMs for the vendor library; the diagonal triangles are one
+         * batched launch of the hand-written SYRK tiles (separate launches of
+         * a few hundred tiles each would leave most of the chip idle in their
+         * last round). */
+        const int64_t smin = lower_split_min(g, Prec<T>::f64);
+        int64_t nb = 2;
+        while ((g.M + nb - 1) / nb >= smin) nb *= 2;
+        const int64_t S = ((g.M + nb - 1) / nb + 255) / 256 * 256;
+        nb = (g.M + S - 1) / S;
+        auto panel = [&](int64_t z) {
+            GemmArgs r = g;
+            r.lower = false;
+            r.M = std::min<int64_t>(S, g.M - z * S);
+            r.N = z * S;
+            r.va = g.va.sub(z * S, 0);
+            r.vc = g.vc.sub(z * S, 0);
+            return r;
+        };
+        bool lib = nb > 1;
+        for (int64_t z = 1; z < nb && lib; ++z) lib = blas_route(h, panel(z), m, Prec<T>::f64) != 0;
+        if (lib) {
+            for (int64_t z = 1; z < nb; ++z) RFP_TRY(run_gemm<T>(h, panel(z), m));
+            GemmArgs t = g;
+            t.M = t.N = S;
+            t.batch = (int)nb;
+            t.a_bs = S * g.va.rs;
+            t.b_bs = S * g.vb.rs;
+            t.c_bs = S * (g.vc.rs + g.vc.cs);
+            t.m_last = g.M - (nb - 1) * S;
+            GemmArgs last = g;
+            last.M = last.N = t.m_last;
+            const double fl = (double)(nb - 1) * gemm_flops(t) + gemm_flops(last);
+            const double by = (double)(nb - 1) * gemm_bytes<T>(t, m) + gemm_bytes<T>(last, m);
+            return timed(h, cls, fl, by, [&] { return Prec<T>::gemm(t, m, h->stream); });
         }
     }
     if (const int route = blas_route(h, g, m, Prec<T>::f64))

@@ -215,8 +215,14 @@ struct GemmArgs {
      * row blocks at view vcs — the shadow of a block that has become final */
     unsigned short *c_sh_hi, *c_sh_lo;
     RfpView vcs;
+    /* batch > 1 (lower-triangular outputs, hand-written kernels only): the
+     * same product on `batch` diagonal blocks, block z at element offsets
+     * z * a_bs, z * b_bs, z * c_bs from block 0, the last block of order
+     * m_last <= M */
+    int batch;
+    int64_t a_bs, b_bs, c_bs, m_last;
 };
-static inline GemmArgs gemm_args() { GemmArgs g{}; g.splits = 1; return g; }
+static inline GemmArgs gemm_args() { GemmArgs g{}; g.splits = 1; g.batch = 1; return g; }
 hipError_t launch_gemm_f32(const GemmArgs &g, RfpMath math, hipStream_t s);
 /* BF16 row-block GEMM with direct global -> LDS loads, bt x bt tiles (256 or
  * 128); returns hipErrorNotSupported when the operands do not meet its
