@@ -139,7 +139,14 @@ static hipError_t timed(curfpHandle_t h, int cls, double flops, double bytes, F 
     (void)hipEventRecord(e0, h->stream);
     hipError_t e = launch();
     (void)hipEventRecord(e1, h->stream);
-    h->prof_pending.push_back(RfpProfRecord{cls, flops, bytes, e0, e1});
+    if (e == hipSuccess) {
+        h->prof_pending.push_back(RfpProfRecord{cls, flops, bytes, e0, e1});
+    } else {
+        /* nothing of this class ran (a failed launch, or a library that
+         * declined the problem and handed it back) */
+        h->prof_pool.push_back(e0);
+        h->prof_pool.push_back(e1);
+    }
     return e;
 }
 
@@ -422,8 +429,14 @@ static hipError_t run_gemm(curfpHandle_t h, const GemmArgs &g, RfpMath m)
             return timed(h, cls, fl, by, [&] { return Prec<T>::gemm(t, m, h->stream); });
         }
     }
-    if (const int route = blas_route(h, g, m, Prec<T>::f64))
-        return timed(h, cls, gemm_flops(g), gemm_bytes<T>(g, m), [&] { return blas_gemm(h, g, m, route); });
+    if (const int route = blas_route(h, g, m, Prec<T>::f64)) {
+        /* hipErrorNotSupported: the library has no plan for this problem (no
+         * algorithm, or its state could not be created) — the hand-written
+         * kernels take it */
+        const hipError_t e =
+            timed(h, cls, gemm_flops(g), gemm_bytes<T>(g, m), [&] { return blas_gemm(h, g, m, route); });
+        if (e != hipErrorNotSupported) return e;
+    }
     return timed(h, cls, gemm_flops(g), gemm_bytes<T>(g, m), [&] { return Prec<T>::gemm(g, m, h->stream); });
 }
 
