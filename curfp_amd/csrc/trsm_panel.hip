@@ -341,14 +341,44 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
         }
     }
     if (p.sh_hi) {
-        for (int idx = tid; idx < RM * wm; idx += 256) {
-            const int r = idx / wm, c = idx % wm;
-            if (r >= rows || c >= p.m) continue;
-            const float v = Xs[r * XP + c];
-            const int64_t so = p.S.off + (r0 + r) * p.S.rs + (int64_t)c * p.S.cs;
-            const unsigned short hb = bf16_rne(v);
-            p.sh_hi[so] = hb;
-            if (p.sh_lo) p.sh_lo[so] = bf16_rne(v - __uint_as_float((unsigned)hb << 16));
+        if (p.S.cs == 1 && p.S.rs % 8 == 0 && p.S.off % 8 == 0) {
+            /* shadow rows k-contiguous and 16-byte aligned: eight columns per
+             * thread and store (two 16-byte LDS reads), a scalar tail past m */
+            constexpr int CH = MAXQ * LF / 8;   /* 8-column chunks per slab row */
+#pragma unroll
+            for (int e = 0; e < RM * CH / 256; ++e) {
+                const int idx = tid + 256 * e, r = idx / CH, c = (idx % CH) * 8;
+                if (r >= rows || c >= p.m) continue;
+                const f32x4v x0 = *(const f32x4v *)(Xs + r * XP + c), x1 = *(const f32x4v *)(Xs + r * XP + c + 4);
+                const float x[8] = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+                const int64_t so = p.S.off + (r0 + r) * p.S.rs + c;
+                if (c + 8 <= p.m) {
+                    bf16x8v hv, lv;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) {
+                        hv[q] = (__bf16)x[q];   /* nearest even */
+                        lv[q] = (__bf16)(x[q] - (float)hv[q]);
+                    }
+                    *reinterpret_cast<bf16x8v *>(p.sh_hi + so) = hv;
+                    if (p.sh_lo) *reinterpret_cast<bf16x8v *>(p.sh_lo + so) = lv;
+                } else {
+                    for (int q = 0; q < p.m - c; ++q) {
+                        const unsigned short hb = bf16_rne(x[q]);
+                        p.sh_hi[so + q] = hb;
+                        if (p.sh_lo) p.sh_lo[so + q] = bf16_rne(x[q] - __uint_as_float((unsigned)hb << 16));
+                    }
+                }
+            }
+        } else {
+            for (int idx = tid; idx < RM * wm; idx += 256) {
+                const int r = idx / wm, c = idx % wm;
+                if (r >= rows || c >= p.m) continue;
+                const float v = Xs[r * XP + c];
+                const int64_t so = p.S.off + (r0 + r) * p.S.rs + (int64_t)c * p.S.cs;
+                const unsigned short hb = bf16_rne(v);
+                p.sh_hi[so] = hb;
+                if (p.sh_lo) p.sh_lo[so] = bf16_rne(v - __uint_as_float((unsigned)hb << 16));
+            }
         }
     }
 }

@@ -174,8 +174,20 @@ struct Replay {
                         }
                     }
                 }
-            /* shadow writes (:341-351) */
-            if (bf16)
+            /* shadow writes (the end of trsm_panel_kernel): eight columns per
+             * thread and store along 16-byte aligned k-contiguous shadow rows,
+             * else one element per thread and store */
+            if (bf16 && p.S.cs == 1 && p.S.rs % 8 == 0 && p.S.off % 8 == 0) {
+                const int CH = MAXQ * LF / 8;
+                for (int tid = 0; tid < THREADS; ++tid)
+                    for (int e = 0; e < RM * CH / THREADS; ++e) {
+                        const int idx = tid + THREADS * e, r = idx / CH, c = (idx % CH) * 8;
+                        if (r >= rows || c >= m) continue;
+                        coord(r < RM && c + 7 < XP, "shadow source outside LDS", r, c);
+                        const int cnt = c + 8 <= m ? 8 : m - c;
+                        for (int q = 0; q < cnt; ++q) at(shadow, p.S.at(r0 + r, c + q), "BF16 shadow");
+                    }
+            } else if (bf16) {
                 for (int tid = 0; tid < THREADS; ++tid)
                     for (int idx = tid; idx < RM * wm; idx += THREADS) {
                         const int r = idx / wm, c = idx % wm;
@@ -183,6 +195,7 @@ struct Replay {
                         coord(r < RM && c < XP, "shadow source outside LDS", r, c);
                         at(shadow, p.S.at(r0 + r, c), "BF16 shadow");
                     }
+            }
         }
     }
 };
