@@ -49,8 +49,7 @@ __device__ __forceinline__ unsigned short bf16_rne(float x)
     return __builtin_bit_cast(unsigned short, b);
 }
 
-/* The wave's two 16-column tiles of a 128-column leaf: w and 7 - w, so the
- * triangular inverse products give every wave the same number of k chunks. */
+/* The wave's two 16-column tiles of a 128-column leaf: w and 7 - w. */
 __device__ __forceinline__ int col_tile(int wave, int c) { return c == 0 ? wave : 7 - wave; }
 
 struct Ops {
@@ -78,9 +77,10 @@ __device__ __forceinline__ void ops_load(Ops &o, const float *Xs, int x0, const 
 }
 
 /* acc[t][c] (rows 16 t .., columns col_tile(wave, c) ..) += slab(:, x0 ..
- * x0 + 128) * tile^T; TRI: the tile is lower triangular (an inverse), so the
- * k chunks past a column tile's diagonal are skipped */
-template <bool BKC, bool TRI>
+ * x0 + 128) * tile^T.  The inverse tiles are lower triangular; skipping the
+ * k chunks past a column tile's diagonal measured slower than multiplying
+ * their zeros (0.4 ms per factorisation at n = 65536), so every chunk runs. */
+template <bool BKC>
 __device__ __forceinline__ void mma_panel(f32x4v (&acc)[2][2], const float *Xs, int x0, const float *Ls, int wave,
                                           int lane)
 {
@@ -90,14 +90,13 @@ __device__ __forceinline__ void mma_panel(f32x4v (&acc)[2][2], const float *Xs, 
     for (int q = 0; q < LF / 16; ++q) {
         if (q + 1 < LF / 16) ops_load<BKC>(o[(q + 1) & 1], Xs, x0, Ls, q + 1, wave, lane);
         const Ops &cur = o[q & 1];
-        /* four independent accumulators interleaved (wave-uniform skips) */
-        const bool n0 = !TRI || q <= col_tile(wave, 0), n1 = !TRI || q <= col_tile(wave, 1);
+        /* four independent accumulators interleaved */
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
 #pragma unroll
             for (int t = 0; t < 2; ++t) {
-                if (n0) acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[t][s], cur.b[0][s], acc[t][0], 0, 0, 0);
-                if (n1) acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[t][s], cur.b[1][s], acc[t][1], 0, 0, 0);
+                acc[t][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[t][s], cur.b[0][s], acc[t][0], 0, 0, 0);
+                acc[t][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(cur.a[t][s], cur.b[1][s], acc[t][1], 0, 0, 0);
             }
         }
     }
@@ -275,12 +274,12 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
                 if (kc) mma_panel_bf16<true>(acc, Xs, pp * LF, Ls, wave, lane);
                 else    mma_panel_bf16<false>(acc, Xs, pp * LF, Ls, wave, lane);
             } else {
-                if (kc) mma_panel<true, false>(acc, Xs, pp * LF, Ls, wave, lane);
-                else    mma_panel<false, false>(acc, Xs, pp * LF, Ls, wave, lane);
+                if (kc) mma_panel<true>(acc, Xs, pp * LF, Ls, wave, lane);
+                else    mma_panel<false>(acc, Xs, pp * LF, Ls, wave, lane);
             }
         } else {
             /* X_q = R_q inv(L_qq)^T; the slab's leaf q is rewritten once every wave has read it */
-            mma_panel<true, true>(acc, Xs, q * LF, Ls, wave, lane);
+            mma_panel<true>(acc, Xs, q * LF, Ls, wave, lane);
             lds_barrier();
 #pragma unroll
             for (int t = 0; t < 2; ++t)
