@@ -140,8 +140,47 @@ __device__ __forceinline__ void mma_panel_bf16(f32x4v (&acc)[2][2], const float 
     }
 }
 
+/* mma_panel with both operands split into BF16 high and low parts, three
+ * v_mfma_f32_16x16x32_bf16 per 32-wide k chunk (hi hi^T + hi lo^T + lo hi^T,
+ * FP32 accumulation): products to ~2^-16 of FP32's, the lo lo^T term (2^-16
+ * relative) dropped.  Same LDS reads as mma_panel_bf16. */
+template <bool BKC>
+__device__ __forceinline__ void mma_panel_bf16x3(f32x4v (&acc)[2][2], const float *Xs, int x0, const float *Ls,
+                                                 int wave, int lane)
+{
+    Ops lo, hi;
+    ops_load<BKC>(lo, Xs, x0, Ls, 0, wave, lane);
+    ops_load<BKC>(hi, Xs, x0, Ls, 1, wave, lane);
+#pragma unroll
+    for (int q = 0; q < LF / 32; ++q) {
+        bf16x8v ah[2], al[2], bh[2], bl[2];
+        auto split = [](const f32x4v &x0, const f32x4v &x1, bf16x8v &h, bf16x8v &l) {
+            h = round_bf16(x0, x1);
+            const f32x4v r0 = {x0[0] - (float)h[0], x0[1] - (float)h[1], x0[2] - (float)h[2], x0[3] - (float)h[3]};
+            const f32x4v r1 = {x1[0] - (float)h[4], x1[1] - (float)h[5], x1[2] - (float)h[6], x1[3] - (float)h[7]};
+            l = round_bf16(r0, r1);
+        };
+#pragma unroll
+        for (int t = 0; t < 2; ++t) split(lo.a[t], hi.a[t], ah[t], al[t]);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) split(lo.b[c], hi.b[c], bh[c], bl[c]);
+        if (q + 1 < LF / 32) {
+            ops_load<BKC>(lo, Xs, x0, Ls, 2 * q + 2, wave, lane);
+            ops_load<BKC>(hi, Xs, x0, Ls, 2 * q + 3, wave, lane);
+        }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[t], bh[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bl[c], acc[t][c], 0, 0, 0);
+                acc[t][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[t], bh[c], acc[t][c], 0, 0, 0);
+            }
+    }
+}
+
 /* BF16: the off-diagonal products X_p L_qp^T are BF16 products (FP32
- * accumulation); the inverse products stay FP32. */
+ * accumulation) and the inverse products split-BF16 (mma_panel_bf16x3). */
 template <bool BF16>
 __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
 {
@@ -279,7 +318,8 @@ __global__ __launch_bounds__(256) void trsm_panel_kernel(PanelArgs p)
             }
         } else {
             /* X_q = R_q inv(L_qq)^T; the slab's leaf q is rewritten once every wave has read it */
-            mma_panel<true>(acc, Xs, q * LF, Ls, wave, lane);
+            if (BF16) mma_panel_bf16x3<true>(acc, Xs, q * LF, Ls, wave, lane);
+            else      mma_panel<true>(acc, Xs, q * LF, Ls, wave, lane);
             lds_barrier();
 #pragma unroll
             for (int t = 0; t < 2; ++t)
