@@ -691,6 +691,37 @@ __device__ __forceinline__ void mma_bf16t(f32x16_t &acc, const float *A, const f
     }
 }
 
+/* mma_bf16t with both operands split into BF16 high and low parts (three
+ * MFMAs per 16-wide k step: hi hi^T + hi lo^T + lo hi^T, products to ~2^-16
+ * relative): the split-BF16 inverse products of the BF16-mode panels. */
+__device__ __forceinline__ void mma_bf16x3t(f32x16_t &acc, const float *A, const float *B, int lane)
+{
+    const int r = lane & 31, h = lane >> 5;
+    float a[2][8], b[2][8];
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            a[t][e] = A[r * LP + 16 * t + 8 * h + e];
+            b[t][e] = B[r * LP + 16 * t + 8 * h + e];
+        }
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        bf16x8_t ah, al, bh, bl;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            ah[e] = (__bf16)a[t][e];
+            al[e] = (__bf16)(a[t][e] - (float)ah[e]);
+            bh[e] = (__bf16)b[t][e];
+            bl[e] = (__bf16)(b[t][e] - (float)bh[e]);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+    }
+}
+
 /*
  * Two leaves of a 128 < m <= 256 diagonal block in one launch: the
  * recursion's potrf(m) = potrf(L00); X := C10 L00^-T; C11 -= X X^T;
@@ -701,7 +732,8 @@ __device__ __forceinline__ void mma_bf16t(f32x16_t &acc, const float *A, const f
  * and multiplied by the inverse still in X (the panel's rows, written to the
  * view and, in the BF16 modes, to the BF16 shadow view sv); C11 is loaded
  * into X, updated with the panel's product (BF16-rounded operands when bf16,
- * like the BF16 updates), and factored as leaf 1 with the roles of the two
+ * like the BF16 updates; the panel itself split-BF16 then, like the BF16-mode
+ * panel TRSM), and factored as leaf 1 with the roles of the two
  * LDS arrays exchanged.  info as LAPACK's (first failing column of the
  * block).
  */
@@ -762,8 +794,11 @@ __global__ __launch_bounds__(256) void potrf_pair_kernel(float *base, RfpView v,
         for (int jb = 0; jb < NBLK; ++jb) {
             zero(acc[jb]);
 #pragma unroll
-            for (int kb = 0; kb <= jb; ++kb)
-                mma(acc[jb], L + (wave * BS) * LP + kb * BS, LP, 1, X + (jb * BS) * LP + kb * BS, 1, LP, lane);
+            for (int kb = 0; kb <= jb; ++kb) {
+                const float *Ak = L + (wave * BS) * LP + kb * BS, *Bk = X + (jb * BS) * LP + kb * BS;
+                if (bf16) mma_bf16x3t(acc[jb], Ak, Bk, lane);   /* as the BF16-mode panel TRSM */
+                else      mma(acc[jb], Ak, LP, 1, Bk, 1, LP, lane);
+            }
         }
         wave_sync();   /* the slab is the wave's own: its reads are done before it is overwritten */
 #pragma unroll
