@@ -739,12 +739,9 @@ struct Factor {
     hipError_t trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return hipSuccess;
-        if (!Prec<T>::f64 && m <= trsm_panel_max() && (!sh_hi || mb <= panel_shape::BF16_MAX_ROWS)) {
+        if (!Prec<T>::f64 && m <= trsm_panel_max()) {
             /* FP32: the last levels of the recursion in one launch, each
-             * workgroup substituting a slab of rows through all leaves.  In the
-             * BF16 modes panels taller than 16384 rows keep the recursion,
-             * whose off-diagonal updates are BF16 GEMMs from the shadow
-             * (measured faster there than the panel's FP32 products). */
+             * workgroup substituting a slab of rows through all leaves. */
             const double fl = (double)mb * (double)m * (double)m;
             const double by = (double)mb * m * 4.0 * 2.0 + (double)m * m * 4.0 + (sh_hi ? (double)mb * m * 2.0 : 0.0);
             return timed(h, CURFP_KERNEL_TRSM, fl, by, [&] {

@@ -13,7 +13,6 @@ constexpr int THREADS = 256;                        /* threads per workgroup */
 constexpr int XP = MAXQ * LF + 4;                   /* LDS pitch of the slab (== 4 mod 64) */
 constexpr int SP = LF + 4;                          /* LDS pitch of a staged tile */
 constexpr int SL = RM * MAXQ * LF / 4 / THREADS;    /* 16-byte slab pieces per thread */
-constexpr int BF16_MAX_ROWS = 16384;                /* BF16 modes: taller panels keep the recursion */
 }  // namespace panel_shape
 
 /* FP64 panel (trsm_panel64.hip): 64-row leaves */

@@ -19,9 +19,9 @@
  *   panel_replay chain n transr uplo bf16 [all]
  *       every panel launch of spftrf at order n (Factor::potrf / potrf2 /
  *       trsm_rt of rfp_drivers.hip), for the FP32 (bf16 = 0), BF16 (1) or
- *       split-BF16 (3) modes (BF16: shadow writes, panels of at most
- *       BF16_MAX_ROWS rows; FP32 and BF16: no panel launch inside a block of
- *       up to two leaves, which is one pair launch);
+ *       split-BF16 (3) modes (BF16: shadow writes; FP32 and BF16: no panel
+ *       launch inside a block of up to two leaves, which is one pair
+ *       launch);
  *       workgroups 0, 1 and the last two of each launch (every address of a
  *       workgroup is its first row r0 = 32 * blockIdx.x times the row stride
  *       plus a pattern that depends on blockIdx.x only through the row count
@@ -300,7 +300,7 @@ struct Chain {
     void trsm_rt(RfpView B, RfpView Bs, int64_t mb, RfpView L, RfpView Ls, int64_t m, int64_t slot)
     {
         if (m <= 0 || mb <= 0) return;
-        if (m <= MAXQ * LF && (!bf16 || mb <= BF16_MAX_ROWS)) {
+        if (m <= MAXQ * LF) {
             rp.launch(Launch{B, L, Bs, mb, m, slot});
             return;
         }
