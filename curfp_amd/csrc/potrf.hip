@@ -628,14 +628,22 @@ __device__ __forceinline__ void leaf_inverse(const float *Lt, float *Xt, float (
     const int lane = tid & 63, wave = tid >> 6;
     if (wave < NBLK - 1) inv_block(Lt, Xt, Tt[wave], NBLK - 1, wave, lane);
     __syncthreads();
-#pragma unroll 1
+    /* each thread's 16 pieces of 4: all LDS reads before the stores */
+    float w[NB * NB / 1024][4];
+#pragma unroll
+    for (int e = 0; e < NB * NB / 1024; ++e) {
+        const int idx = 4 * (tid + 256 * e);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) w[e][u] = Xt[(idx / NB) * LP + idx % NB + u];
+    }
+#pragma unroll
     for (int e = 0; e < NB * NB / 1024; ++e) {
         const int idx = 4 * (tid + 256 * e);
         const int i = idx / NB, j = idx % NB;
-        float w[4];
+        float o[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) w[u] = (i < m && j + u <= i) ? Xt[i * LP + j + u] : 0.f;
-        *reinterpret_cast<float4 *>(inv + idx) = make_float4(w[0], w[1], w[2], w[3]);
+        for (int u = 0; u < 4; ++u) o[u] = (i < m && j + u <= i) ? w[e][u] : 0.f;
+        *reinterpret_cast<float4 *>(inv + idx) = make_float4(o[0], o[1], o[2], o[3]);
     }
 }
 
