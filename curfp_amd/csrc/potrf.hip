@@ -603,19 +603,28 @@ __device__ __forceinline__ int leaf_factor(float *Lt, float *Xt, float (*Tt)[BS 
 /* The factor's lower triangle (m x m) back to the view */
 __device__ __forceinline__ void leaf_store(const float *Lt, float *base, RfpView v, int m, int tid)
 {
+    /* a thread's 64 elements (every other column of its row, or every other
+     * row of its column): all LDS reads, then the stores of those on or below
+     * the diagonal and inside the tile */
+    float x[PER];
     if (v.rs == 1) {
         const int i = tid % NB, j0 = tid / NB;
-        if (i < m) {
-            float *p = base + v.off + i + (int64_t)j0 * v.cs;
-            const int64_t step = 2 * v.cs;
-            for (int j = j0; j <= i; j += 2, p += step) *p = Lt[i * LP + j];
-        }
+#pragma unroll
+        for (int e = 0; e < PER; ++e) x[e] = Lt[i * LP + j0 + 2 * e];
+        float *p = base + v.off + i + (int64_t)j0 * v.cs;
+        const int64_t step = 2 * v.cs;
+#pragma unroll
+        for (int e = 0; e < PER; ++e)
+            if (i < m && j0 + 2 * e <= i) p[e * step] = x[e];
     } else {
         const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+        for (int e = 0; e < PER; ++e) x[e] = Lt[(i0 + 2 * e) * LP + j];
         float *p = base + v.off + (int64_t)i0 * v.rs + j;
         const int64_t step = 2 * v.rs;
-        for (int i = i0; i < m; i += 2, p += step)
-            if (i >= j) *p = Lt[i * LP + j];
+#pragma unroll
+        for (int e = 0; e < PER; ++e)
+            if (i0 + 2 * e < m && i0 + 2 * e >= j) p[e * step] = x[e];
     }
 }
 
