@@ -824,16 +824,27 @@ __global__ __launch_bounds__(256) void potrf_pair_kernel(float *base, RfpView v,
     __syncthreads();
     /* the panel to the view and the BF16 shadow; C11 (its lower triangle,
      * identity beyond m2) into X, whose inverse of leaf 0 is written out */
-    if (v10.rs == 1) {
-        const int i = tid % NB, j0 = tid / NB;
-        if (i < m2) {
+    {
+        /* all LDS reads of the thread's 64 elements, then the stores of the
+         * rows inside the panel */
+        float x[PER];
+        if (v10.rs == 1) {
+            const int i = tid % NB, j0 = tid / NB;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) x[e] = L[i * LP + j0 + 2 * e];
             float *p = base + v10.off + i + (int64_t)j0 * v10.cs;
-            for (int j = j0; j < NB; j += 2, p += 2 * v10.cs) *p = L[i * LP + j];
+#pragma unroll
+            for (int e = 0; e < PER; ++e)
+                if (i < m2) p[e * 2 * v10.cs] = x[e];
+        } else {
+            const int j = tid % NB, i0 = tid / NB;
+#pragma unroll
+            for (int e = 0; e < PER; ++e) x[e] = L[(i0 + 2 * e) * LP + j];
+            float *p = base + v10.off + (int64_t)i0 * v10.rs + j;
+#pragma unroll
+            for (int e = 0; e < PER; ++e)
+                if (i0 + 2 * e < m2) p[e * 2 * v10.rs] = x[e];
         }
-    } else {
-        const int j = tid % NB, i0 = tid / NB;
-        float *p = base + v10.off + (int64_t)i0 * v10.rs + j;
-        for (int i = i0; i < m2; i += 2, p += 2 * v10.rs) *p = L[i * LP + j];
     }
     if (sh_hi) {
         const RfpView s10 = sv.sub(NB, 0);
