@@ -959,12 +959,24 @@ __global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView
     if (*info != INT_MAX) return;          /* an earlier leaf already failed */
 
     /* the lower triangle into s, zero above, identity beyond m; loads run
-     * along the view's contiguous direction */
+     * along the view's contiguous direction.  Every load is issued before
+     * the first LDS store: an element outside the lower triangle of the
+     * m x m tile is loaded from (0, 0) instead (always inside the view) and
+     * not used, so no load sits in a branch of its own. */
+    double x[NB / 2];
+#pragma unroll
     for (int e = 0; e < NB / 2; ++e) {
         const int o = 2 * e + wave;        /* the other index, two per step */
         const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
+        const bool in = r < m && c < m && r >= c;
+        x[e] = base[v.off + (in ? (int64_t)r * v.rs + (int64_t)c * v.cs : 0)];
+    }
+#pragma unroll
+    for (int e = 0; e < NB / 2; ++e) {
+        const int o = 2 * e + wave;
+        const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
         double val = r == c ? 1.0 : 0.0;
-        if (r < m && c < m) val = r >= c ? base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] : 0.0;
+        if (r < m && c < m) val = r >= c ? x[e] : 0.0;
         s[r * P + c] = val;
     }
     __syncthreads();
@@ -1010,10 +1022,20 @@ __global__ __launch_bounds__(128) void potrf_leaf64_kernel(double *base, RfpView
     }
     __syncthreads();
     const int cend = fail >= 0 ? fail : m;
-    for (int e = 0; e < NB / 2; ++e) {
-        const int o = 2 * e + wave;
-        const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
-        if (r < m && c < cend && r >= c) base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] = s[r * P + c];
+    {
+        double y[NB / 2];
+#pragma unroll
+        for (int e = 0; e < NB / 2; ++e) {
+            const int o = 2 * e + wave;
+            const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
+            y[e] = s[r * P + c];
+        }
+#pragma unroll
+        for (int e = 0; e < NB / 2; ++e) {
+            const int o = 2 * e + wave;
+            const int r = v.rs == 1 ? lane : o, c = v.rs == 1 ? o : lane;
+            if (r < m && c < cend && r >= c) base[v.off + (int64_t)r * v.rs + (int64_t)c * v.cs] = y[e];
+        }
     }
     if (fail >= 0) {
         if (tid == 0) atomicMin(info, info_base + fail + 1);

@@ -102,13 +102,26 @@ __global__ __launch_bounds__(THREADS) void trsm_panel64_kernel(Panel64Args p)
     const int nq = (p.m + LF - 1) / LF, nsteps = nq * (nq + 1) / 2;
     const int ct[2] = {wave, 3 - wave};
 
-    /* the slab: rows r0.., columns 0..m-1 (zero beyond mb and m) */
+    /* the slab: rows r0.., columns 0..m-1 (zero beyond mb and m).  All of a
+     * thread's loads are issued before its LDS stores: an element outside
+     * the panel is loaded from X(r0, 0) instead (inside the panel) and not
+     * used, so no load waits in a branch of its own. */
     const bool xc = p.X.cs == 1;
-#pragma unroll 4
-    for (int e = 0; e < SE; ++e) {
-        int r, c;
-        slab_elem(tid, e, xc, r, c);
-        Xs[r * XP + c] = (r0 + r < p.mb && c < p.m) ? p.a[p.X.at(r0 + r, c)] : 0.0;
+    {
+        double xv[SE];
+#pragma unroll
+        for (int e = 0; e < SE; ++e) {
+            int r, c;
+            slab_elem(tid, e, xc, r, c);
+            const bool in = r0 + r < p.mb && c < p.m;
+            xv[e] = p.a[in ? p.X.at(r0 + r, c) : p.X.at(r0, 0)];
+        }
+#pragma unroll
+        for (int e = 0; e < SE; ++e) {
+            int r, c;
+            slab_elem(tid, e, xc, r, c);
+            Xs[r * XP + c] = (r0 + r < p.mb && c < p.m) ? xv[e] : 0.0;
+        }
     }
     double v[TE];
     tile_load(p, 0, tid, v);
