@@ -117,14 +117,17 @@ __device__ __forceinline__ void tile_coords(const KParams &p, int64_t t, int64_t
 /*
  * Load a ROWS x BK slice (rows r0.., k k0..) of view (off, rs, cs) into
  * registers.  KC: the view is contiguous along k (cs == 1), otherwise along
- * rows (rs == 1).  Each thread holds NV vectors of Vec<T>::N elements.
+ * rows (rs == 1).  Each thread holds NV vectors of Vec<S>::N elements in the
+ * storage type S; the conversion to the compute type T happens in store(), so
+ * that no load is waited for where it is issued (a conversion next to its
+ * load would be).
  */
 template <typename S, typename T, int ROWS, int BK, bool KC>
 struct SliceLoader {
     static constexpr int VN = Vec<S>::N;
     static constexpr int NV = ROWS * BK / (256 * VN);
     static_assert(NV >= 1, "slice too small");
-    T v[NV][VN];
+    S v[NV][VN];
 
     __device__ __forceinline__ void load(const S *base, int64_t off, int64_t rs, int64_t cs,
                                          int64_t r0, int64_t k0, int64_t R, int64_t K, int tid)
@@ -142,20 +145,20 @@ struct SliceLoader {
                 if (gr < R && gk + VN <= K) {
                     V x = *(const V *)p;
 #pragma unroll
-                    for (int c = 0; c < VN; ++c) v[e][c] = (T)x[c];
+                    for (int c = 0; c < VN; ++c) v[e][c] = x[c];
                 } else {
 #pragma unroll
-                    for (int c = 0; c < VN; ++c) v[e][c] = (gr < R && gk + c < K) ? (T)p[c] : T(0);
+                    for (int c = 0; c < VN; ++c) v[e][c] = (gr < R && gk + c < K) ? p[c] : S(0);
                 }
             } else {
                 const S *p = base + off + gr + gk * cs;
                 if (gk < K && gr + VN <= R) {
                     V x = *(const V *)p;
 #pragma unroll
-                    for (int c = 0; c < VN; ++c) v[e][c] = (T)x[c];
+                    for (int c = 0; c < VN; ++c) v[e][c] = x[c];
                 } else {
 #pragma unroll
-                    for (int c = 0; c < VN; ++c) v[e][c] = (gk < K && gr + c < R) ? (T)p[c] : T(0);
+                    for (int c = 0; c < VN; ++c) v[e][c] = (gk < K && gr + c < R) ? p[c] : S(0);
                 }
             }
         }
@@ -173,8 +176,8 @@ struct SliceLoader {
             else    { rr = (id % (ROWS / VN)) * VN; kq = id / (ROWS / VN); }
 #pragma unroll
             for (int c = 0; c < VN; ++c) {
-                if (KC) s[(kq + c) * LDS + rr] = v[e][c];
-                else    s[kq * LDS + rr + c] = v[e][c];
+                if (KC) s[(kq + c) * LDS + rr] = (T)v[e][c];
+                else    s[kq * LDS + rr + c] = (T)v[e][c];
             }
         }
     }
