@@ -275,7 +275,7 @@ __global__ __launch_bounds__(256) void potrf_tile_kernel(T *base, RfpView v, int
 }
 
 template <typename T, int NB>
-__global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView v, int64_t m, T *inv)
+__device__ __forceinline__ void trtri_tile(const T *base, RfpView v, int64_t m, T *inv, int64_t t)
 {
     constexpr int P = NB + 1;
     constexpr int RB = NB / 16;
@@ -283,7 +283,6 @@ __global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView
     __shared__ T colbuf[2][NB];
     __shared__ T rowbuf[2][NB];
     const int tid = threadIdx.x;
-    const int64_t t = blockIdx.x;
     const int64_t r0 = t * NB;
     const int mt = (int)((m - r0) < NB ? (m - r0) : NB);
     RegTile<T, NB> rt;
@@ -297,6 +296,18 @@ __global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView
     rt.template to_lds<P>(s, x);
     __syncthreads();
     write_inverse<T, NB, P>(s, inv + t * NB * NB, mt, tid);
+}
+
+/* Leaf inverses of up to two views in one launch: workgroups 0 .. nt1 - 1
+ * invert view 1's leaves, the rest view 2's (two workgroups fit on a CU, so
+ * both views' leaves run together instead of one launch after the other). */
+template <typename T, int NB>
+__global__ __launch_bounds__(256) void trtri_tiles_kernel(const T *base, RfpView v1, int64_t m1, T *inv1,
+                                                          int64_t nt1, RfpView v2, int64_t m2, T *inv2)
+{
+    const int64_t t = blockIdx.x;
+    if (t < nt1) trtri_tile<T, NB>(base, v1, m1, inv1, t);
+    else         trtri_tile<T, NB>(base, v2, m2, inv2, t - nt1);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1065,12 +1076,14 @@ static hipError_t potrf_tile(T *base, RfpView v, int m, int *info, int info_base
 }
 
 template <typename T>
-static hipError_t trtri_tiles(const T *base, RfpView v, int64_t m, T *inv, hipStream_t s)
+static hipError_t trtri_tiles(const T *base, RfpView v1, int64_t m1, T *inv1, RfpView v2, int64_t m2, T *inv2,
+                              hipStream_t s)
 {
     constexpr int NB = LeafSize<T>::value;
-    if (m <= 0) return hipSuccess;
-    const int64_t nt = (m + NB - 1) / NB;
-    hipLaunchKernelGGL((trtri_tiles_kernel<T, NB>), dim3((unsigned)nt), dim3(256), 0, s, base, v, m, inv);
+    const int64_t nt1 = m1 > 0 ? (m1 + NB - 1) / NB : 0, nt2 = m2 > 0 ? (m2 + NB - 1) / NB : 0;
+    if (nt1 + nt2 == 0) return hipSuccess;
+    hipLaunchKernelGGL((trtri_tiles_kernel<T, NB>), dim3((unsigned)(nt1 + nt2)), dim3(256), 0, s, base, v1, m1, inv1,
+                       nt1, v2, m2, inv2);
     return hipGetLastError();
 }
 
@@ -1091,6 +1104,12 @@ hipError_t launch_potrf_pair_f32(float *base, RfpView v, int m, int *info, int i
 hipError_t launch_potrf_tile_f64(double *base, RfpView v, int m, int *info, int info_base, double *inv, hipStream_t s)
 { return potrf_tile<double>(base, v, m, info, info_base, inv, s); }
 hipError_t launch_trtri_tiles_f32(const float *base, RfpView v, int64_t m, float *inv, hipStream_t s)
-{ return trtri_tiles<float>(base, v, m, inv, s); }
+{ return trtri_tiles<float>(base, v, m, inv, v, 0, inv, s); }
 hipError_t launch_trtri_tiles_f64(const double *base, RfpView v, int64_t m, double *inv, hipStream_t s)
-{ return trtri_tiles<double>(base, v, m, inv, s); }
+{ return trtri_tiles<double>(base, v, m, inv, v, 0, inv, s); }
+hipError_t launch_trtri_tiles2_f32(const float *base, RfpView v1, int64_t m1, float *inv1, RfpView v2, int64_t m2,
+                                   float *inv2, hipStream_t s)
+{ return trtri_tiles<float>(base, v1, m1, inv1, v2, m2, inv2, s); }
+hipError_t launch_trtri_tiles2_f64(const double *base, RfpView v1, int64_t m1, double *inv1, RfpView v2, int64_t m2,
+                                   double *inv2, hipStream_t s)
+{ return trtri_tiles<double>(base, v1, m1, inv1, v2, m2, inv2, s); }

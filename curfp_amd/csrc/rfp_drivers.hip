@@ -322,6 +322,9 @@ template <> struct Prec<float> {
     { return launch_potrf_tile_f32(a, v, m, info, base, inv, s); }
     static hipError_t trtri(const float *a, RfpView v, int64_t m, float *inv, hipStream_t s)
     { return launch_trtri_tiles_f32(a, v, m, inv, s); }
+    static hipError_t trtri2(const float *a, RfpView v1, int64_t m1, float *inv1, RfpView v2, int64_t m2, float *inv2,
+                             hipStream_t s)
+    { return launch_trtri_tiles2_f32(a, v1, m1, inv1, v2, m2, inv2, s); }
     static hipError_t trttf(const RfpGeom &g, bool lo, const float *A, int64_t lda, float *arf, hipStream_t s)
     { return launch_trttf_f32(g, lo, A, lda, arf, s); }
     static hipError_t tfttr(const RfpGeom &g, bool lo, const float *arf, float *A, int64_t lda, hipStream_t s)
@@ -336,6 +339,9 @@ template <> struct Prec<double> {
     { return launch_potrf_tile_f64(a, v, m, info, base, inv, s); }
     static hipError_t trtri(const double *a, RfpView v, int64_t m, double *inv, hipStream_t s)
     { return launch_trtri_tiles_f64(a, v, m, inv, s); }
+    static hipError_t trtri2(const double *a, RfpView v1, int64_t m1, double *inv1, RfpView v2, int64_t m2,
+                             double *inv2, hipStream_t s)
+    { return launch_trtri_tiles2_f64(a, v1, m1, inv1, v2, m2, inv2, s); }
     static hipError_t trttf(const RfpGeom &g, bool lo, const double *A, int64_t lda, double *arf, hipStream_t s)
     { return launch_trttf_f64(g, lo, A, lda, arf, s); }
     static hipError_t tfttr(const RfpGeom &g, bool lo, const double *arf, double *A, int64_t lda, hipStream_t s)
@@ -965,10 +971,10 @@ static hipError_t pftrs_core(curfpHandle_t h, const RfpGeom &g, const T *A, T *B
     const int64_t leaf = rfp_leaf(Prec<T>::f64);
     const int64_t s1 = (g.n1 + leaf - 1) / leaf;
     if (!prepared) {
-        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n1 * leaf * leaf / 3.0, 0.0,
-                      [&] { return Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream); }));
-        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n2 * leaf * leaf / 3.0, 0.0,
-                      [&] { return Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream); }));
+        /* every diagonal leaf's inverse, both views in one launch */
+        RFP_TRY(timed(h, CURFP_KERNEL_TRTRI, (double)g.n * leaf * leaf / 3.0, 0.0, [&] {
+            return Prec<T>::trtri2(A, g.v11, g.n1, inv, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream);
+        }));
     }
     if (!Prec<T>::f64 && nrhs <= 64 * SWEEP_MAX_GROUPS) {
         /* FP32: one sweep launch per direction over all 128-row blocks; the
@@ -1363,11 +1369,9 @@ static curfpStatus_t pftri_impl(curfpHandle_t h, curfpOperation_t transr, curfpF
     hipError_t e = rfp_workspace(h, (size_t)(s1 + s2) * leaf * leaf * sizeof(T), &ws);
     if (e != hipSuccess) return st_of(e);
     T *inv = (T *)ws;
-    /* inverses of every diagonal leaf, two launches */
+    /* inverses of every diagonal leaf, both views in one launch */
     e = timed(h, CURFP_KERNEL_TRTRI, (double)n * leaf * leaf / 3.0, 0.0, [&] {
-        hipError_t r = Prec<T>::trtri(A, g.v11, g.n1, inv, h->stream);
-        if (r == hipSuccess) r = Prec<T>::trtri(A, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream);
-        return r;
+        return Prec<T>::trtri2(A, g.v11, g.n1, inv, g.v22, g.n2, inv + s1 * leaf * leaf, h->stream);
     });
     if (e != hipSuccess) return st_of(e);
     Inverter<T> iv{h, A, inv, leaf};

@@ -295,6 +295,11 @@ hipError_t launch_potrf_pair_f32(float *base, RfpView v, int m, int *info, int i
  */
 hipError_t launch_trtri_tiles_f32(const float *base, RfpView v, int64_t m, float *inv, hipStream_t s);
 hipError_t launch_trtri_tiles_f64(const double *base, RfpView v, int64_t m, double *inv, hipStream_t s);
+/* The same for two views in one launch (their leaves' inverses at inv1, inv2) */
+hipError_t launch_trtri_tiles2_f32(const float *base, RfpView v1, int64_t m1, float *inv1, RfpView v2, int64_t m2,
+                                   float *inv2, hipStream_t s);
+hipError_t launch_trtri_tiles2_f64(const double *base, RfpView v1, int64_t m1, double *inv1, RfpView v2, int64_t m2,
+                                   double *inv2, hipStream_t s);
 
 /*
  * FP32 block triangular solve (trsv_block.hip): B (m x nrhs view) :=
