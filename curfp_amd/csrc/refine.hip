@@ -9,48 +9,48 @@
 
 namespace {
 
+/* The element-wise kernels below run on a 2-D grid: x over the n rows, y
+ * over the columns (strided when nrhs exceeds the grid), so no thread
+ * divides a flat index by n. */
 __global__ void cast_f64_f32_kernel(int64_t n, int64_t nrhs, const double *src, int64_t lds, float *dst,
                                     int64_t ldd)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * nrhs) return;
-    const int64_t i = idx % n, c = idx / n;
-    dst[i + c * ldd] = (float)src[i + c * lds];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t c = blockIdx.y; c < nrhs; c += gridDim.y) dst[i + c * ldd] = (float)src[i + c * lds];
 }
 
 /* X = D (accumulate = false) or X += D, D in FP32, X in FP64 */
 __global__ void acc_f32_f64_kernel(int64_t n, int64_t nrhs, const float *D, int64_t ldd, double *X, int64_t ldx,
                                    int accumulate)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * nrhs) return;
-    const int64_t i = idx % n, c = idx / n;
-    const double d = (double)D[i + c * ldd];
-    double *x = X + i + c * ldx;
-    *x = accumulate ? *x + d : d;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t c = blockIdx.y; c < nrhs; c += gridDim.y) {
+        const double d = (double)D[i + c * ldd];
+        double *x = X + i + c * ldx;
+        *x = accumulate ? *x + d : d;
+    }
 }
 
 /* Y = X, both n x nrhs column-major FP64 */
 __global__ void copy_f64_kernel(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *Y, int64_t ldy)
 {
-    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= n * nrhs) return;
-    const int64_t i = idx % n, c = idx / n;
-    Y[i + c * ldy] = X[i + c * ldx];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (int64_t c = blockIdx.y; c < nrhs; c += gridDim.y) Y[i + c * ldy] = X[i + c * ldx];
 }
 
 /* out[slot] += sum of squares of an n x nrhs column-major FP64 matrix */
 __global__ void sumsq_kernel(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out)
 {
     __shared__ double red[256];
-    const int64_t total = n * nrhs;
     double s = 0.0;
-    for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
-         idx += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = idx % n, c = idx / n;
-        const double v = X[i + c * ldx];
-        s += v * v;
-    }
+    for (int64_t c = blockIdx.y; c < nrhs; c += gridDim.y)
+        for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+            const double v = X[i + c * ldx];
+            s += v * v;
+        }
     red[threadIdx.x] = s;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
@@ -198,6 +198,16 @@ __global__ void zero_f64_kernel(double *p, int n)
     for (int i = (int)threadIdx.x; i < n; i += (int)blockDim.x) p[i] = 0.0;
 }
 
+/* 2-D grid for an n x nrhs element-wise pass: rows over x, columns over y */
+inline dim3 grid_2d(int64_t n, int64_t nrhs, int64_t xcap = 1LL << 30)
+{
+    int64_t bx = (n + 255) / 256;
+    if (bx > xcap) bx = xcap;
+    if (bx < 1) bx = 1;
+    const int64_t by = nrhs < 1 ? 1 : (nrhs > 65535 ? 65535 : nrhs);
+    return dim3((unsigned)bx, (unsigned)by);
+}
+
 inline unsigned blocks_for(int64_t total, int64_t cap = 1 << 20)
 {
     int64_t b = (total + 255) / 256;
@@ -211,7 +221,7 @@ inline unsigned blocks_for(int64_t total, int64_t cap = 1 << 20)
 hipError_t launch_cast_f64_f32(int64_t n, int64_t nrhs, const double *src, int64_t lds, float *dst, int64_t ldd,
                                hipStream_t s)
 {
-    hipLaunchKernelGGL(cast_f64_f32_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, src,
+    hipLaunchKernelGGL(cast_f64_f32_kernel, grid_2d(n, nrhs), dim3(256), 0, s, n, nrhs, src,
                        lds, dst, ldd);
     return hipGetLastError();
 }
@@ -219,7 +229,7 @@ hipError_t launch_cast_f64_f32(int64_t n, int64_t nrhs, const double *src, int64
 hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t ldd, double *X, int64_t ldx,
                               bool accumulate, hipStream_t s)
 {
-    hipLaunchKernelGGL(acc_f32_f64_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, D, ldd,
+    hipLaunchKernelGGL(acc_f32_f64_kernel, grid_2d(n, nrhs), dim3(256), 0, s, n, nrhs, D, ldd,
                        X, ldx, accumulate ? 1 : 0);
     return hipGetLastError();
 }
@@ -227,14 +237,14 @@ hipError_t launch_acc_f32_f64(int64_t n, int64_t nrhs, const float *D, int64_t l
 hipError_t launch_copy_f64(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *Y, int64_t ldy,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(copy_f64_kernel, dim3(blocks_for(n * nrhs, 1LL << 30)), dim3(256), 0, s, n, nrhs, X, ldx, Y,
+    hipLaunchKernelGGL(copy_f64_kernel, grid_2d(n, nrhs), dim3(256), 0, s, n, nrhs, X, ldx, Y,
                        ldy);
     return hipGetLastError();
 }
 
 hipError_t launch_sumsq(int64_t n, int64_t nrhs, const double *X, int64_t ldx, double *out, hipStream_t s)
 {
-    hipLaunchKernelGGL(sumsq_kernel, dim3(blocks_for(n * nrhs, 2048)), dim3(256), 0, s, n, nrhs, X, ldx, out);
+    hipLaunchKernelGGL(sumsq_kernel, grid_2d(n, nrhs < 64 ? nrhs : 64, 32), dim3(256), 0, s, n, nrhs, X, ldx, out);
     return hipGetLastError();
 }
 
