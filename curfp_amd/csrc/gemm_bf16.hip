@@ -485,15 +485,45 @@ __global__ __launch_bounds__(256) void to_bf16_rows_kernel(const float *A, RfpVi
                                                            int64_t Kp, int64_t ldo, unsigned short *hi,
                                                            unsigned short *lo)
 {
-    const int64_t total = rows * Kp;
-    const bool along_k = av.cs == 1;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = along_k ? t / Kp : t % rows;
-        const int64_t p = along_k ? t % Kp : t / rows;
-        const float x = p < K ? A[av.off + i * av.rs + p * av.cs] : 0.f;
-        const unsigned short h = bf16_bits(x);
-        hi[i * ldo + p] = h;
-        if (lo) lo[i * ldo + p] = bf16_bits(x - to_f(h));
+    /* a thread converts 4 consecutive k of a row: x over the k groups, y
+     * over the rows (strided; a short Kp puts several rows in a workgroup);
+     * 16-byte loads where k is contiguous and the group lies inside K,
+     * 8-byte stores where the output rows allow */
+    typedef float f32x4a __attribute__((ext_vector_type(4), aligned(4)));
+    typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+    const int64_t p4 = 4 * ((int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (p4 >= Kp) return;
+    const bool vec_in = av.cs == 1 && p4 + 4 <= K;
+    const bool vec_out = ldo % 4 == 0 && p4 + 4 <= Kp && ((uintptr_t)hi & 7) == 0 && ((uintptr_t)lo & 7) == 0;
+    for (int64_t i = (int64_t)blockIdx.y * blockDim.y + threadIdx.y; i < rows; i += (int64_t)gridDim.y * blockDim.y) {
+        const float *src = A + av.off + i * av.rs + p4 * av.cs;
+        float x[4];
+        if (vec_in) {
+            const f32x4a t = *(const f32x4a *)src;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = t[u];
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) x[u] = p4 + u < K ? src[u * av.cs] : 0.f;
+        }
+        u16x4 h, l;
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            h[u] = bf16_bits(x[u]);
+            l[u] = bf16_bits(x[u] - to_f((unsigned short)h[u]));
+        }
+        unsigned short *ho = hi + i * ldo + p4, *lw = lo ? lo + i * ldo + p4 : nullptr;
+        if (vec_out) {
+            *(u16x4 *)ho = h;
+            if (lw) *(u16x4 *)lw = l;
+        } else {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (p4 + u < Kp) {
+                    ho[u] = h[u];
+                    if (lw) lw[u] = l[u];
+                }
+        }
     }
 }
 
@@ -538,9 +568,12 @@ hipError_t launch_to_bf16_rows(const float *A, RfpView av, int64_t rows, int64_t
         hipLaunchKernelGGL(to_bf16_rows_t_kernel, grid, dim3(256), 0, s, A, av, rows, K, Kp, ldo, hi, lo);
         return hipGetLastError();
     }
-    const int64_t total = rows * Kp;
-    const unsigned blocks = (unsigned)std::min<int64_t>((total + 255) / 256, 16384);
-    hipLaunchKernelGGL(to_bf16_rows_kernel, dim3(blocks), dim3(256), 0, s, A, av, rows, K, Kp, ldo, hi, lo);
+    unsigned bx = 1;
+    while (bx < 256 && 4 * (int64_t)bx < Kp) bx *= 2;
+    const dim3 block(bx, 256 / bx);
+    const dim3 grid((unsigned)((Kp + 4 * bx - 1) / (4 * bx)),
+                    (unsigned)std::min<int64_t>((rows + block.y - 1) / block.y, 16384));
+    hipLaunchKernelGGL(to_bf16_rows_kernel, grid, block, 0, s, A, av, rows, K, Kp, ldo, hi, lo);
     return hipGetLastError();
 }
 
